@@ -1,0 +1,199 @@
+"""ctypes binding of libclspmv.so (the C ABI declared in include/clspmv.h).
+
+The library is loaded from clspmv_amd/lib/libclspmv.so (built in-tree by
+``make -C clspmv_amd/csrc`` or ``__graft_entry__.build()``).  Loading fails
+loudly when it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libclspmv.so")
+
+OK = 0
+E_INVALID, E_TOO_LARGE, E_PADDING, E_HIP, E_NOMEM, E_IO, E_SORT, E_UNSUPPORTED, E_NODEVICE = (
+    -1, -2, -3, -4, -5, -6, -7, -8, -9)
+ERROR_NAMES = {0: "OK", -1: "E_INVALID", -2: "E_TOO_LARGE", -3: "E_PADDING", -4: "E_HIP",
+               -5: "E_NOMEM", -6: "E_IO", -7: "E_SORT", -8: "E_UNSUPPORTED", -9: "E_NODEVICE"}
+
+FMT_CSR_SCALAR, FMT_CSR_VECTOR, FMT_BDIA, FMT_DIA, FMT_ELL, FMT_COO = 1, 2, 3, 4, 5, 6
+FMT_BELL, FMT_BCSR, FMT_SELL, FMT_SBELL, FMT_BSR = 7, 8, 9, 10, 11
+
+GPU_ALIGNMENT = 32
+MAX_MEM_OBJ = 268435456
+
+_IP = C.POINTER(C.c_int)
+_FP = C.POINTER(C.c_float)
+
+
+class MatInfo(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("nnz", C.c_int)]
+
+
+class Coo(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("coo_row_id", _IP), ("coo_col_id", _IP), ("coo_data", _FP)]
+
+
+class Csr(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("csr_row_ptr", _IP), ("csr_col_id", _IP), ("csr_data", _FP)]
+
+
+class Dia(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("dia_num", C.c_int), ("dia_length", C.c_int),
+                ("dia_length_aligned", C.c_int), ("dia_offsets", _IP), ("dia_data", _FP)]
+
+
+class Bdia(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("bdia_band_num", C.c_int), ("bdia_length", C.c_int),
+                ("bdia_length_aligned", C.c_int), ("bdia_offsets", _IP), ("bdia_bptr", _IP),
+                ("bdia_data", _FP)]
+
+
+class Ell(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("ell_num", C.c_int), ("ell_height_aligned", C.c_int),
+                ("ell_col_id", _IP), ("ell_data", _FP)]
+
+
+class Sell(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("sell_slice_height", C.c_uint), ("sell_slice_num", C.c_int),
+                ("sell_slice_ptr", _IP), ("sell_col_id", _IP), ("sell_data", _FP)]
+
+
+class B4Csr(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("b4csr_bwidth", C.c_uint), ("b4csr_bheight", C.c_uint),
+                ("b4csr_row_num", C.c_int), ("b4csr_aligned_size", C.c_int), ("b4csr_block_num", C.c_int),
+                ("b4csr_row_ptr", _IP), ("b4csr_col_id", _IP), ("b4csr_data", _FP)]
+
+
+class B4Ell(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("b4ell_bwidth", C.c_uint), ("b4ell_bheight", C.c_uint),
+                ("b4ell_row_num", C.c_int), ("b4ell_block_num", C.c_int), ("b4ell_height_aligned", C.c_int),
+                ("b4ell_float4_aligned", C.c_int), ("b4ell_col_id", _IP), ("b4ell_data", _FP)]
+
+
+class Sbell(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("sbell_bwidth", C.c_uint), ("sbell_bheight", C.c_uint),
+                ("sbell_slice_height", C.c_uint), ("sbell_slice_num", C.c_int), ("sbell_row_num", C.c_int),
+                ("sbell_slice_ptr", _IP), ("sbell_col_id", _IP), ("sbell_data", _FP)]
+
+
+class Bsr(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("bsr_bwidth", C.c_uint), ("bsr_bheight", C.c_uint),
+                ("bsr_row_num", C.c_int), ("bsr_block_num", C.c_int), ("bsr_block_stride", C.c_int),
+                ("bsr_row_ptr", _IP), ("bsr_col_id", _IP), ("bsr_data", _FP)]
+
+
+class PartInfo(C.Structure):
+    _fields_ = [("format", C.c_int), ("method", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
+                ("nnz", C.c_longlong), ("stored_bytes", C.c_longlong), ("alg_bytes", C.c_longlong),
+                ("launches", C.c_int), ("kernel", C.c_char * 64)]
+
+
+# (name, restype, argtypes)
+_P = C.POINTER
+_SIGS = [
+    ("clspmv_last_error", C.c_char_p, []),
+    ("clspmv_init_coo", None, [_P(Coo)]),
+    ("clspmv_free_coo", None, [_P(Coo)]),
+    ("clspmv_free_csr", None, [_P(Csr)]),
+    ("clspmv_free_dia", None, [_P(Dia)]),
+    ("clspmv_free_bdia", None, [_P(Bdia)]),
+    ("clspmv_free_ell", None, [_P(Ell)]),
+    ("clspmv_free_sell", None, [_P(Sell)]),
+    ("clspmv_free_b4csr", None, [_P(B4Csr)]),
+    ("clspmv_free_b4ell", None, [_P(B4Ell)]),
+    ("clspmv_free_sbell", None, [_P(Sbell)]),
+    ("clspmv_free_bsr", None, [_P(Bsr)]),
+    ("clspmv_alloc_coo", C.c_int, [_P(Coo), C.c_int, C.c_int, C.c_int]),
+    ("clspmv_if_sorted_coo", C.c_int, [_P(Coo)]),
+    ("clspmv_sort_coo", C.c_int, [_P(Coo)]),
+    ("clspmv_aligned_length", C.c_int, [C.c_int, C.c_int]),
+    ("clspmv_coo2csr", C.c_int, [_P(Coo), _P(Csr)]),
+    ("clspmv_coo2dia", C.c_int, [_P(Coo), _P(Dia), C.c_int]),
+    ("clspmv_coo2bdia", C.c_int, [_P(Coo), _P(Bdia), C.c_int]),
+    ("clspmv_coo2ell", C.c_int, [_P(Coo), _P(Ell), C.c_int, C.c_int]),
+    ("clspmv_coo2sell", C.c_int, [_P(Coo), _P(Sell), C.c_uint]),
+    ("clspmv_coo2b4csr", C.c_int, [_P(Coo), _P(B4Csr), C.c_uint, C.c_uint, C.c_int]),
+    ("clspmv_coo2b4ell", C.c_int, [_P(Coo), _P(B4Ell), C.c_uint, C.c_uint, C.c_int, C.c_int]),
+    ("clspmv_coo2sbell", C.c_int, [_P(Coo), _P(Sbell), C.c_uint, C.c_uint, C.c_uint]),
+    ("clspmv_coo2bsr", C.c_int, [_P(Coo), _P(Bsr), C.c_uint, C.c_uint]),
+    ("clspmv_pad_csr", C.c_int, [_P(Csr), _P(Csr), C.c_int]),
+    ("clspmv_read_mmf", C.c_int, [C.c_char_p, _P(Coo)]),
+    ("clspmv_write_mmf", C.c_int, [C.c_char_p, _P(Coo)]),
+    ("clspmv_write_coo_bin", C.c_int, [C.c_char_p, _P(Coo)]),
+    ("clspmv_read_coo_bin", C.c_int, [C.c_char_p, _P(Coo)]),
+    ("clspmv_gen_laplacian2d", C.c_int, [_P(Coo), C.c_int, C.c_int]),
+    ("clspmv_gen_stencil27", C.c_int, [_P(Coo), C.c_int, C.c_uint64]),
+    ("clspmv_gen_uniform_rows", C.c_int, [_P(Coo), C.c_int, C.c_int, C.c_uint64]),
+    ("clspmv_gen_zipf_rows", C.c_int, [_P(Coo), C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint64]),
+    ("clspmv_gen_block_rows", C.c_int, [_P(Coo), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]),
+    ("clspmv_gen_cocktail_composite", C.c_int, [_P(Coo), C.c_int, C.c_uint64]),
+    ("clspmv_gen_vector", C.c_int, [_FP, C.c_int, C.c_uint64]),
+    ("clspmv_init", C.c_int, [C.c_int, _P(C.c_void_p)]),
+    ("clspmv_finalize", C.c_int, [C.c_void_p]),
+    ("clspmv_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("clspmv_synchronize", C.c_int, [C.c_void_p]),
+    ("clspmv_device_alloc", C.c_int, [C.c_void_p, C.c_size_t, _P(C.c_void_p)]),
+    ("clspmv_device_free", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("clspmv_memcpy_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("clspmv_memcpy_d2h", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("clspmv_memcpy_d2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("clspmv_event_elapsed_begin", C.c_int, [C.c_void_p]),
+    ("clspmv_event_elapsed_end", C.c_int, [C.c_void_p, _FP]),
+    ("clspmv_part_csr_scalar", C.c_int, [C.c_void_p, _P(Csr), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_csr_vector", C.c_int, [C.c_void_p, _P(Csr), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_dia", C.c_int, [C.c_void_p, _P(Dia), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_bdia", C.c_int, [C.c_void_p, _P(Bdia), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_ell", C.c_int, [C.c_void_p, _P(Ell), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_sell", C.c_int, [C.c_void_p, _P(Sell), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_coo", C.c_int, [C.c_void_p, _P(Coo), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_bell", C.c_int, [C.c_void_p, _P(B4Ell), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_sbell", C.c_int, [C.c_void_p, _P(Sbell), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_bcsr", C.c_int, [C.c_void_p, _P(B4Csr), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_bsr", C.c_int, [C.c_void_p, _P(Bsr), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_spmv", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    ("clspmv_part_get_info", C.c_int, [C.c_void_p, _P(PartInfo)]),
+    ("clspmv_part_free", C.c_int, [C.c_void_p]),
+    ("clspmv_probe_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _FP]),
+    ("clspmv_probe_launch", C.c_int, [C.c_void_p, C.c_int, _FP]),
+]
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libclspmv.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C clspmv_amd/csrc` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names this binding declares (all must be exported by the library)."""
+    return [s[0] for s in _SIGS]
+
+
+class ClspmvError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        msg = lib().clspmv_last_error()
+        detail = msg.decode() if msg else ""
+        super().__init__(f"{what} failed: {ERROR_NAMES.get(code, code)} {detail}")
+        self.code = code
+
+
+def check(code: int, what: str) -> int:
+    if code != OK:
+        raise ClspmvError(code, what)
+    return code

@@ -1,0 +1,271 @@
+// Blocked-format SpMV kernels (y += A*x) for gfx950: BELL (b4ell), SBELL,
+// BCSR (b4csr) with the reference's float4-column layouts and block shapes
+// bh in {1,2,4,8} x bw in {4,8}, plus a native r x c BSR (e.g. 3x3).
+//
+//  * BELL / SBELL: one lane per block row (A/kernels/spmv_bell.cl,
+//    spmv_sbell.cl use the same mapping); the value layouts make lane r's
+//    float4 for (slot, w4, h) adjacent to lane r+1's, so each dwordx4 load
+//    of a wave reads 1 KiB of contiguous HBM.  x is read as float4 at the
+//    block's float4 column (guarded at the right matrix edge).
+//  * BCSR: the reference gives every block row a 128-thread workgroup with an
+//    LDS tree reduction (A/kernels/spmv_bcsr.cl:4-44).  Here a group of L
+//    lanes (L from the mean blocks per block row) strides the blocks of one
+//    block row and reduces with shuffles, so short block rows do not idle a
+//    whole workgroup.
+//  * BSR r x c: element-major values data[e*stride + i]; same L-lane scheme.
+#include "kcommon.hpp"
+#include "launch.hpp"
+
+namespace clspmv {
+namespace dev {
+
+// x as float4 at float4-column c4 (elements 4*c4 .. 4*c4+3), zero past `cols`.
+__device__ __forceinline__ float4 x4load(const float* __restrict__ x, int c4, int cols) {
+    const int c = c4 * 4;
+    if (c + 3 < cols) return *reinterpret_cast<const float4*>(x + c);
+    float4 r;
+    r.x = (c < cols) ? x[c] : 0.f;
+    r.y = (c + 1 < cols) ? x[c + 1] : 0.f;
+    r.z = (c + 2 < cols) ? x[c + 2] : 0.f;
+    r.w = 0.f;
+    return r;
+}
+
+__device__ __forceinline__ float dot4(float4 a, float4 b, float acc) {
+    acc = fmaf(a.x, b.x, acc);
+    acc = fmaf(a.y, b.y, acc);
+    acc = fmaf(a.z, b.z, acc);
+    return fmaf(a.w, b.w, acc);
+}
+
+template <int BH, int W4>
+__global__ __launch_bounds__(256) void bell_kernel(const int* __restrict__ col, const float4* __restrict__ data,
+                                                   int Hc, int F4q, int slots, int brows,
+                                                   const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                   int cols) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= brows) return;
+    float acc[BH];
+#pragma unroll
+    for (int h = 0; h < BH; ++h) acc[h] = 0.f;
+    for (int j = 0; j < slots; ++j) {
+        const int c4 = ld_stream(col + (size_t)j * Hc + r);
+        float4 xv[W4];
+#pragma unroll
+        for (int w4 = 0; w4 < W4; ++w4) xv[w4] = x4load(x, c4 + w4, cols);
+        const float4* dp = data + (size_t)j * (W4 * BH) * F4q + r;
+#pragma unroll
+        for (int w4 = 0; w4 < W4; ++w4)
+#pragma unroll
+            for (int h = 0; h < BH; ++h) acc[h] = dot4(ld_stream(dp + (size_t)(w4 * BH + h) * F4q), xv[w4], acc[h]);
+    }
+#pragma unroll
+    for (int h = 0; h < BH; ++h) {
+        const int row = r * BH + h;
+        if (row < rows) y[row] += acc[h];
+    }
+}
+
+template <int BH, int W4>
+__global__ __launch_bounds__(256) void sbell_kernel(const int* __restrict__ slice_ptr, const int* __restrict__ col,
+                                                    const float4* __restrict__ data, int S, int brows,
+                                                    const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                    int cols) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= brows) return;
+    const int s = r / S, t = r - s * S;
+    const int b = slice_ptr[s];
+    const int width = (slice_ptr[s + 1] - b) / S;
+    float acc[BH];
+#pragma unroll
+    for (int h = 0; h < BH; ++h) acc[h] = 0.f;
+    // float4 index of (slot j, w4, h, t) = b*W4*BH + j*(S*W4*BH) + w4*(S*BH) + h*S + t
+    const float4* dbase = data + (size_t)b * (W4 * BH) + t;
+    for (int j = 0; j < width; ++j) {
+        const int c4 = ld_stream(col + b + t + (size_t)j * S);
+        float4 xv[W4];
+#pragma unroll
+        for (int w4 = 0; w4 < W4; ++w4) xv[w4] = x4load(x, c4 + w4, cols);
+        const float4* dp = dbase + (size_t)j * (S * W4 * BH);
+#pragma unroll
+        for (int w4 = 0; w4 < W4; ++w4)
+#pragma unroll
+            for (int h = 0; h < BH; ++h) acc[h] = dot4(ld_stream(dp + (size_t)(w4 * BH + h) * S), xv[w4], acc[h]);
+    }
+#pragma unroll
+    for (int h = 0; h < BH; ++h) {
+        const int row = r * BH + h;
+        if (row < rows) y[row] += acc[h];
+    }
+}
+
+template <int BH, int W4, int L>
+__global__ __launch_bounds__(256) void bcsr_kernel(const int* __restrict__ rp, const int* __restrict__ col,
+                                                   const float4* __restrict__ data, int N4, int brows,
+                                                   const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                   int cols) {
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & (L - 1);
+    const int r = gid / L;
+    if (r >= brows) return;
+    float acc[BH];
+#pragma unroll
+    for (int h = 0; h < BH; ++h) acc[h] = 0.f;
+    const int s = rp[r], e = rp[r + 1];
+    for (int i = s + lane; i < e; i += L) {
+        const int c4 = ld_stream(col + i);
+#pragma unroll
+        for (int w4 = 0; w4 < W4; ++w4) {
+            const float4 xv = x4load(x, c4 + w4, cols);
+#pragma unroll
+            for (int h = 0; h < BH; ++h) acc[h] = dot4(ld_stream(data + (size_t)(w4 * BH + h) * N4 + i), xv, acc[h]);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < BH; ++h) acc[h] = group_sum<L>(acc[h]);
+    if (lane == 0 && e > s) {
+#pragma unroll
+        for (int h = 0; h < BH; ++h) {
+            const int row = r * BH + h;
+            if (row < rows) y[row] += acc[h];
+        }
+    }
+}
+
+template <int BH, int BW, int L>
+__global__ __launch_bounds__(256) void bsr_kernel(const int* __restrict__ rp, const int* __restrict__ col,
+                                                  const float* __restrict__ data, int stride, int brows,
+                                                  const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                  int cols) {
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & (L - 1);
+    const int r = gid / L;
+    if (r >= brows) return;
+    float acc[BH];
+#pragma unroll
+    for (int h = 0; h < BH; ++h) acc[h] = 0.f;
+    const int s = rp[r], e = rp[r + 1];
+    for (int i = s + lane; i < e; i += L) {
+        const int c = ld_stream(col + i);
+        float xv[BW];
+#pragma unroll
+        for (int w = 0; w < BW; ++w) xv[w] = (c + w < cols) ? x[c + w] : 0.f;
+#pragma unroll
+        for (int h = 0; h < BH; ++h)
+#pragma unroll
+            for (int w = 0; w < BW; ++w) acc[h] = fmaf(ld_stream(data + (size_t)(h * BW + w) * stride + i), xv[w], acc[h]);
+    }
+#pragma unroll
+    for (int h = 0; h < BH; ++h) acc[h] = group_sum<L>(acc[h]);
+    if (lane == 0 && e > s) {
+#pragma unroll
+        for (int h = 0; h < BH; ++h) {
+            const int row = r * BH + h;
+            if (row < rows) y[row] += acc[h];
+        }
+    }
+}
+
+__global__ void copy_probe_kernel(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n4; i += stride) dst[i] = ld_stream(src + i);
+}
+
+__global__ void empty_kernel() {}
+
+}  // namespace dev
+
+#define BH_W4_SWITCH(BH, W4, MACRO)     \
+    switch (BH * 10 + W4) {             \
+        case 11: MACRO(1, 1); break;    \
+        case 21: MACRO(2, 1); break;    \
+        case 41: MACRO(4, 1); break;    \
+        case 81: MACRO(8, 1); break;    \
+        case 12: MACRO(1, 2); break;    \
+        case 22: MACRO(2, 2); break;    \
+        case 42: MACRO(4, 2); break;    \
+        case 82: MACRO(8, 2); break;    \
+        default: return (int)hipErrorInvalidValue; \
+    }
+
+int launch_bell(const DevBell& m, const float* x, float* y, hipStream_t st) {
+    if (m.brows == 0) return 0;
+    dim3 grid((m.brows + 255) / 256);
+#define L_BELL(BH, W4)                                                                                        \
+    hipLaunchKernelGGL((dev::bell_kernel<BH, W4>), grid, dim3(256), 0, st, m.col,                             \
+                       reinterpret_cast<const float4*>(m.data), m.Hc, m.F4 / 4, m.slots, m.brows, x, y, m.rows, \
+                       m.cols)
+    BH_W4_SWITCH(m.bh, m.w4n, L_BELL)
+#undef L_BELL
+    return (int)hipGetLastError();
+}
+
+int launch_sbell(const DevSbell& m, const float* x, float* y, hipStream_t st) {
+    if (m.brows == 0) return 0;
+    dim3 grid((m.brows + 255) / 256);
+#define L_SBELL(BH, W4)                                                                                          \
+    hipLaunchKernelGGL((dev::sbell_kernel<BH, W4>), grid, dim3(256), 0, st, m.slice_ptr, m.col,                  \
+                       reinterpret_cast<const float4*>(m.data), m.S, m.brows, x, y, m.rows, m.cols)
+    BH_W4_SWITCH(m.bh, m.w4n, L_SBELL)
+#undef L_SBELL
+    return (int)hipGetLastError();
+}
+
+template <int BH, int W4>
+static void launch_bcsr_l(const DevBcsr& m, const float* x, float* y, hipStream_t st) {
+    const long long threads = (long long)m.brows * m.lanes;
+    dim3 grid((unsigned)((threads + 255) / 256));
+    const float4* d4 = reinterpret_cast<const float4*>(m.data);
+    switch (m.lanes) {
+        case 2: hipLaunchKernelGGL((dev::bcsr_kernel<BH, W4, 2>), grid, dim3(256), 0, st, m.row_ptr, m.col, d4, m.N4, m.brows, x, y, m.rows, m.cols); break;
+        case 4: hipLaunchKernelGGL((dev::bcsr_kernel<BH, W4, 4>), grid, dim3(256), 0, st, m.row_ptr, m.col, d4, m.N4, m.brows, x, y, m.rows, m.cols); break;
+        case 8: hipLaunchKernelGGL((dev::bcsr_kernel<BH, W4, 8>), grid, dim3(256), 0, st, m.row_ptr, m.col, d4, m.N4, m.brows, x, y, m.rows, m.cols); break;
+        default: hipLaunchKernelGGL((dev::bcsr_kernel<BH, W4, 16>), grid, dim3(256), 0, st, m.row_ptr, m.col, d4, m.N4, m.brows, x, y, m.rows, m.cols); break;
+    }
+}
+
+int launch_bcsr(const DevBcsr& m, const float* x, float* y, hipStream_t st) {
+    if (m.brows == 0) return 0;
+#define L_BCSR(BH, W4) launch_bcsr_l<BH, W4>(m, x, y, st)
+    BH_W4_SWITCH(m.bh, m.w4n, L_BCSR)
+#undef L_BCSR
+    return (int)hipGetLastError();
+}
+
+template <int BH, int BW>
+static void launch_bsr_l(const DevBsr& m, const float* x, float* y, hipStream_t st) {
+    const long long threads = (long long)m.brows * m.lanes;
+    dim3 grid((unsigned)((threads + 255) / 256));
+    switch (m.lanes) {
+        case 2: hipLaunchKernelGGL((dev::bsr_kernel<BH, BW, 2>), grid, dim3(256), 0, st, m.row_ptr, m.col, m.data, m.stride, m.brows, x, y, m.rows, m.cols); break;
+        case 4: hipLaunchKernelGGL((dev::bsr_kernel<BH, BW, 4>), grid, dim3(256), 0, st, m.row_ptr, m.col, m.data, m.stride, m.brows, x, y, m.rows, m.cols); break;
+        case 8: hipLaunchKernelGGL((dev::bsr_kernel<BH, BW, 8>), grid, dim3(256), 0, st, m.row_ptr, m.col, m.data, m.stride, m.brows, x, y, m.rows, m.cols); break;
+        default: hipLaunchKernelGGL((dev::bsr_kernel<BH, BW, 16>), grid, dim3(256), 0, st, m.row_ptr, m.col, m.data, m.stride, m.brows, x, y, m.rows, m.cols); break;
+    }
+}
+
+int launch_bsr(const DevBsr& m, const float* x, float* y, hipStream_t st) {
+    if (m.brows == 0) return 0;
+    switch (m.bh * 10 + m.bw) {
+        case 22: launch_bsr_l<2, 2>(m, x, y, st); break;
+        case 33: launch_bsr_l<3, 3>(m, x, y, st); break;
+        case 44: launch_bsr_l<4, 4>(m, x, y, st); break;
+        case 55: launch_bsr_l<5, 5>(m, x, y, st); break;
+        case 66: launch_bsr_l<6, 6>(m, x, y, st); break;
+        default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_copy_probe(const float4* src, float4* dst, size_t n4, hipStream_t st) {
+    hipLaunchKernelGGL(dev::copy_probe_kernel, dim3(4096), dim3(256), 0, st, src, dst, n4);
+    return (int)hipGetLastError();
+}
+
+int launch_empty(hipStream_t st) {
+    hipLaunchKernelGGL(dev::empty_kernel, dim3(1), dim3(64), 0, st);
+    return (int)hipGetLastError();
+}
+
+}  // namespace clspmv

@@ -1,0 +1,164 @@
+// DIA and BDIA SpMV kernels (y += A*x) for gfx950.
+//
+// Both use the DIA data layout data[d*L + row] (A/include/matrix_storage.h
+// :30-47), so a wave's load of one diagonal is 256 contiguous bytes.  x is
+// read with a bounds test instead of the reference's zero-padded copy
+// (vec_extended, A/src/spmv_cocktail.cpp:253-301).
+//   DIA  method 0: one row per lane (A/kernels/spmv_dia.cl:4-28 "gpu_dia");
+//        method 1: four rows per lane, dwordx4 data loads ("gpu_dia_v4").
+//   BDIA method 0: one row per lane, diagonals of a band read x directly
+//                  (A/kernels/spmv_bdia.cl:46-79 "gpu_bdia_nlvec");
+//        method 1: the x window of each band (256 + width - 1 values) is
+//                  staged in LDS once per workgroup and shared by the band's
+//                  diagonals ("gpu_bdia", with the barrier the reference
+//                  omits between bands, spmv_bdia.cl:27-41).
+// Offsets are uniform across the workgroup and are read with scalar loads.
+#include "kcommon.hpp"
+#include "launch.hpp"
+
+namespace clspmv {
+namespace dev {
+
+__device__ __forceinline__ float xload(const float* __restrict__ x, int c, int cols) {
+    return (c >= 0 && c < cols) ? x[c] : 0.f;
+}
+
+__global__ __launch_bounds__(256) void dia_row_kernel(const int* __restrict__ offsets, const float* __restrict__ data,
+                                                      int L, int ndiag, const float* __restrict__ x,
+                                                      float* __restrict__ y, int rows, int cols) {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int row = tile * 256 + threadIdx.x;
+    if (row >= rows) return;
+    float acc = y[row];
+    const float* dp = data + row;
+    int d = 0;
+    for (; d + 4 <= ndiag; d += 4) {
+        float v[4], xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[u] = ld_stream(dp + (size_t)(d + u) * L);
+            xv[u] = xload(x, row + offsets[d + u], cols);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = fmaf(v[u], xv[u], acc);
+    }
+    for (; d < ndiag; ++d) acc = fmaf(ld_stream(dp + (size_t)d * L), xload(x, row + offsets[d], cols), acc);
+    y[row] = acc;
+}
+
+__global__ __launch_bounds__(256) void dia_row4_kernel(const int* __restrict__ offsets,
+                                                       const float4* __restrict__ data, int L4, int ndiag,
+                                                       const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                       int cols) {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int r4 = tile * 256 + threadIdx.x;
+    const int row = r4 * 4;
+    if (row >= rows) return;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    for (int d = 0; d < ndiag; ++d) {
+        const int c = row + offsets[d];
+        const float4 v = ld_stream(data + (size_t)d * L4 + r4);
+        float x0, x1, x2, x3;
+        if (c >= 0 && c + 3 < cols) {
+            x0 = x[c];
+            x1 = x[c + 1];
+            x2 = x[c + 2];
+            x3 = x[c + 3];
+        } else {
+            x0 = xload(x, c, cols);
+            x1 = xload(x, c + 1, cols);
+            x2 = xload(x, c + 2, cols);
+            x3 = xload(x, c + 3, cols);
+        }
+        a0 = fmaf(v.x, x0, a0);
+        a1 = fmaf(v.y, x1, a1);
+        a2 = fmaf(v.z, x2, a2);
+        a3 = fmaf(v.w, x3, a3);
+    }
+    if (row + 3 < rows) {
+        float4 yo = *reinterpret_cast<const float4*>(y + row);
+        yo.x += a0;
+        yo.y += a1;
+        yo.z += a2;
+        yo.w += a3;
+        *reinterpret_cast<float4*>(y + row) = yo;
+    } else {
+        y[row] += a0;
+        if (row + 1 < rows) y[row + 1] += a1;
+        if (row + 2 < rows) y[row + 2] += a2;
+    }
+}
+
+__global__ __launch_bounds__(256) void bdia_row_kernel(const int* __restrict__ bptr, const int* __restrict__ offsets,
+                                                       const float* __restrict__ data, int L, int nbands,
+                                                       const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                       int cols) {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int row = tile * 256 + threadIdx.x;
+    if (row >= rows) return;
+    float acc = y[row];
+    for (int b = 0; b < nbands; ++b) {
+        const int s = bptr[b], e = bptr[b + 1];
+        const int c0 = row + offsets[b];
+        const float* dp = data + (size_t)s * L + row;
+        for (int j = 0; j < e - s; ++j) acc = fmaf(ld_stream(dp + (size_t)j * L), xload(x, c0 + j, cols), acc);
+    }
+    y[row] = acc;
+}
+
+// LDS-staged x band window; requires every band width <= 257.
+__global__ __launch_bounds__(256) void bdia_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ offsets,
+                                                       const float* __restrict__ data, int L, int nbands,
+                                                       const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                       int cols) {
+    __shared__ float win[512];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int row0 = tile * 256;
+    const int t = threadIdx.x;
+    const int row = row0 + t;
+    const bool live = row < rows;
+    float acc = live ? y[row] : 0.f;
+    for (int b = 0; b < nbands; ++b) {
+        const int s = bptr[b], e = bptr[b + 1];
+        const int w = e - s;
+        const int c0 = row0 + offsets[b];
+        __syncthreads();  // previous band's readers are done with `win`
+        win[t] = xload(x, c0 + t, cols);
+        if (t < w - 1) win[256 + t] = xload(x, c0 + 256 + t, cols);
+        __syncthreads();
+        if (live) {
+            const float* dp = data + (size_t)s * L + row;
+            for (int j = 0; j < w; ++j) acc = fmaf(ld_stream(dp + (size_t)j * L), win[t + j], acc);
+        }
+    }
+    if (live) y[row] = acc;
+}
+
+}  // namespace dev
+
+int launch_dia(const DevDia& m, int method, const float* x, float* y, hipStream_t st) {
+    if (m.rows == 0) return 0;
+    if (method == 1 && (m.L % 4) == 0) {
+        int r4 = (m.rows + 3) / 4;
+        hipLaunchKernelGGL(dev::dia_row4_kernel, dim3((r4 + 255) / 256), dim3(256), 0, st, m.offsets,
+                           reinterpret_cast<const float4*>(m.data), m.L / 4, m.ndiag, x, y, m.rows, m.cols);
+    } else {
+        hipLaunchKernelGGL(dev::dia_row_kernel, dim3((m.rows + 255) / 256), dim3(256), 0, st, m.offsets, m.data, m.L,
+                           m.ndiag, x, y, m.rows, m.cols);
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_bdia(const DevBdia& m, int method, const float* x, float* y, hipStream_t st) {
+    if (m.rows == 0) return 0;
+    if (method == 1) {
+        hipLaunchKernelGGL(dev::bdia_lds_kernel, dim3((m.rows + 255) / 256), dim3(256), 0, st, m.bptr, m.offsets,
+                           m.data, m.L, m.nbands, x, y, m.rows, m.cols);
+    } else {
+        hipLaunchKernelGGL(dev::bdia_row_kernel, dim3((m.rows + 255) / 256), dim3(256), 0, st, m.bptr, m.offsets,
+                           m.data, m.L, m.nbands, x, y, m.rows, m.cols);
+    }
+    return (int)hipGetLastError();
+}
+
+}  // namespace clspmv

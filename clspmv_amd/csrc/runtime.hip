@@ -1,0 +1,720 @@
+// HIP runtime behind the C ABI: contexts, device buffers, per-format device
+// parts and the probes.  Replaces the reference's OpenCL shim
+// (A/src/oclcommon.cpp:259-466) and the device halves of its single-format
+// drivers (e.g. A/src/spmv_ell.cpp:17-300).
+//
+// Every part validates its index arrays on the host before upload, so a
+// kernel can never read outside x or the format arrays, and every y write is
+// guarded by the matrix height.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "clspmv.h"
+#include "host/internal.hpp"
+#include "kernels/launch.hpp"
+#include "runtime.hpp"
+
+using namespace clspmv;
+
+namespace clspmv {
+
+int hip_status(hipError_t e, const char* what) {
+    if (e == hipSuccess) return CLSPMV_OK;
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return CLSPMV_E_HIP;
+}
+
+int DevBuf::upload(clspmv_ctx_t ctx, const void* host, size_t bytes) {
+    size = bytes;
+    hipError_t e = hipMalloc(&ptr, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return hip_status(e, "hipMalloc");
+    }
+    if (bytes && host) {
+        e = hipMemcpyAsync(ptr, host, bytes, hipMemcpyHostToDevice, ctx->stream());
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream());
+        if (e != hipSuccess) return hip_status(e, "hipMemcpy H2D");
+    }
+    return CLSPMV_OK;
+}
+
+int DevBuf::alloc(size_t bytes) {
+    size = bytes;
+    hipError_t e = hipMalloc(&ptr, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return hip_status(e, "hipMalloc");
+    }
+    return CLSPMV_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    size = 0;
+}
+
+}  // namespace clspmv
+
+struct clspmv_part_s {
+    clspmv_ctx_t ctx = nullptr;
+    clspmv_part_info info{};
+    std::vector<DevBuf> bufs;
+    // one of these is live, per info.format
+    DevEll ell{};
+    DevSell sell{};
+    DevCsr csr{};
+    DevCsrAdaptive csra{};
+    int csr_lanes = 0;
+    DevCoo coo{};
+    DevDia dia{};
+    DevBdia bdia{};
+    DevBell bell{};
+    DevSbell sbell{};
+    DevBcsr bcsr{};
+    DevBsr bsr{};
+
+    void* add(clspmv_ctx_t c, const void* host, size_t bytes, int* rc) {
+        bufs.emplace_back();
+        *rc = bufs.back().upload(c, host, bytes);
+        return bufs.back().ptr;
+    }
+    void* scratch(size_t bytes, int* rc) {
+        bufs.emplace_back();
+        *rc = bufs.back().alloc(bytes);
+        return bufs.back().ptr;
+    }
+    ~clspmv_part_s() {
+        for (auto& b : bufs) b.release();
+    }
+};
+
+namespace {
+
+void fill_info(clspmv_part_t p, int fmt, int method, int rows, int cols, long long nnz, const char* kname,
+               int launches) {
+    p->info.format = fmt;
+    p->info.method = method;
+    p->info.rows = rows;
+    p->info.cols = cols;
+    p->info.nnz = nnz;
+    long long stored = 0;
+    for (auto& b : p->bufs) stored += (long long)b.size;
+    p->info.stored_bytes = stored;
+    p->info.alg_bytes = stored + 4LL * cols + 8LL * rows;
+    p->info.launches = launches;
+    std::snprintf(p->info.kernel, sizeof(p->info.kernel), "%s", kname);
+}
+
+int pow2_floor(int v) {
+    int p = 1;
+    while (p * 2 <= v) p *= 2;
+    return p;
+}
+
+bool cols_ok(const int* c, size_t n, int lo, int hi) {
+    for (size_t i = 0; i < n; ++i)
+        if (c[i] < lo || c[i] >= hi) return false;
+    return true;
+}
+
+template <class F>
+int make_part(clspmv_ctx_t ctx, clspmv_part_t* out, F build) {
+    if (!ctx || !out) return CLSPMV_E_INVALID;
+    clspmv_part_t p = new (std::nothrow) clspmv_part_s();
+    if (!p) return CLSPMV_E_NOMEM;
+    p->ctx = ctx;
+    int rc = build(p);
+    if (rc) {
+        delete p;
+        *out = nullptr;
+        return rc;
+    }
+    *out = p;
+    return CLSPMV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clspmv_init(int device, clspmv_ctx_t* ctx) {
+    if (!ctx) return CLSPMV_E_INVALID;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        set_error("no HIP device visible");
+        return CLSPMV_E_NODEVICE;
+    }
+    if (device < 0 || device >= n) return CLSPMV_E_INVALID;
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_status(e, "hipSetDevice");
+    auto* c = new (std::nothrow) clspmv_ctx_s();
+    if (!c) return CLSPMV_E_NOMEM;
+    c->device = device;
+    // Default priority stream (never a prioritised one).
+    if ((e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking)) != hipSuccess) {
+        delete c;
+        return hip_status(e, "hipStreamCreate");
+    }
+    if ((e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+        (void)hipStreamDestroy(c->own);
+        delete c;
+        return hip_status(e, "hipEventCreate");
+    }
+    *ctx = c;
+    return CLSPMV_OK;
+}
+
+int clspmv_finalize(clspmv_ctx_t ctx) {
+    if (!ctx) return CLSPMV_E_INVALID;
+    (void)hipStreamSynchronize(ctx->stream());
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    return CLSPMV_OK;
+}
+
+int clspmv_set_stream(clspmv_ctx_t ctx, void* stream) {
+    if (!ctx) return CLSPMV_E_INVALID;
+    ctx->ext = reinterpret_cast<hipStream_t>(stream);
+    return CLSPMV_OK;
+}
+
+int clspmv_synchronize(clspmv_ctx_t ctx) {
+    if (!ctx) return CLSPMV_E_INVALID;
+    return hip_status(hipStreamSynchronize(ctx->stream()), "hipStreamSynchronize");
+}
+
+int clspmv_device_alloc(clspmv_ctx_t ctx, size_t bytes, void** dptr) {
+    if (!ctx || !dptr) return CLSPMV_E_INVALID;
+    return hip_status(hipMalloc(dptr, bytes ? bytes : 4), "hipMalloc");
+}
+int clspmv_device_free(clspmv_ctx_t ctx, void* dptr) {
+    if (!ctx) return CLSPMV_E_INVALID;
+    return hip_status(hipFree(dptr), "hipFree");
+}
+int clspmv_memcpy_h2d(clspmv_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return CLSPMV_E_INVALID;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream());
+    return hip_status(e, "hipMemcpy H2D");
+}
+int clspmv_memcpy_d2h(clspmv_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return CLSPMV_E_INVALID;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream());
+    return hip_status(e, "hipMemcpy D2H");
+}
+int clspmv_memcpy_d2d(clspmv_ctx_t ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return CLSPMV_E_INVALID;
+    return hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream()), "hipMemcpy D2D");
+}
+
+int clspmv_event_elapsed_begin(clspmv_ctx_t ctx) {
+    if (!ctx) return CLSPMV_E_INVALID;
+    return hip_status(hipEventRecord(ctx->ev0, ctx->stream()), "hipEventRecord");
+}
+int clspmv_event_elapsed_end(clspmv_ctx_t ctx, float* ms) {
+    if (!ctx || !ms) return CLSPMV_E_INVALID;
+    hipError_t e = hipEventRecord(ctx->ev1, ctx->stream());
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
+    return hip_status(e, "hipEvent timing");
+}
+
+// ---------------------------------------------------------------- parts
+
+int clspmv_part_ell(clspmv_ctx_t ctx, const clspmv_ell_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        const int rows = m->matinfo.height, cols = m->matinfo.width, H = m->ell_height_aligned, K = m->ell_num;
+        if (rows < 0 || H < rows || K < 0 || (H % 4) != 0) return CLSPMV_E_INVALID;
+        // Every stored slot is checked, including the padding rows up to H:
+        // the four-rows-per-lane kernel reads them.
+        if (rows > 0 && K > 0 && !cols_ok(m->ell_col_id, (size_t)H * K, 0, cols)) {
+            set_error("ELL column index out of range");
+            return CLSPMV_E_INVALID;
+        }
+        int rc = 0;
+        size_t n = (size_t)H * K;
+        p->ell.col = (const int*)p->add(ctx, m->ell_col_id, n * 4, &rc);
+        if (rc) return rc;
+        p->ell.val = (const float*)p->add(ctx, m->ell_data, n * 4, &rc);
+        if (rc) return rc;
+        p->ell.H = H;
+        p->ell.ell_num = K;
+        p->ell.rows = rows;
+        int meth = (method == 1 || method == 3) ? 1 : 0;  // 2,3: texture variants of 0,1
+        fill_info(p, CLSPMV_FMT_ELL, meth, rows, cols, m->matinfo.nnz, meth ? "ell_row4" : "ell_row", 1);
+        return CLSPMV_OK;
+    });
+}
+
+int clspmv_part_sell(clspmv_ctx_t ctx, const clspmv_sell_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        const int rows = m->matinfo.height, cols = m->matinfo.width;
+        const int S = (int)m->sell_slice_height, ns = m->sell_slice_num;
+        if (S <= 0 || ns != (rows + S - 1) / S) return CLSPMV_E_INVALID;
+        if (m->sell_slice_ptr[0] != 0) return CLSPMV_E_INVALID;
+        for (int s = 0; s < ns; ++s) {
+            int a = m->sell_slice_ptr[s], b = m->sell_slice_ptr[s + 1];
+            if (b < a || (b - a) % S) return CLSPMV_E_INVALID;
+            int w = (b - a) / S;
+            for (int r = s * S; r < std::min(rows, s * S + S); ++r)
+                for (int j = 0; j < w; ++j) {
+                    int c = m->sell_col_id[(size_t)a + (size_t)j * S + (r - s * S)];
+                    if (c < 0 || c >= cols) {
+                        set_error("SELL column index out of range");
+                        return CLSPMV_E_INVALID;
+                    }
+                }
+        }
+        int rc = 0;
+        size_t total = (size_t)m->sell_slice_ptr[ns];
+        p->sell.slice_ptr = (const int*)p->add(ctx, m->sell_slice_ptr, ((size_t)ns + 1) * 4, &rc);
+        if (rc) return rc;
+        p->sell.col = (const int*)p->add(ctx, m->sell_col_id, total * 4, &rc);
+        if (rc) return rc;
+        p->sell.val = (const float*)p->add(ctx, m->sell_data, total * 4, &rc);
+        if (rc) return rc;
+        p->sell.S = S;
+        p->sell.rows = rows;
+        fill_info(p, CLSPMV_FMT_SELL, method & 1, rows, cols, m->matinfo.nnz, "sell_row", 1);
+        return CLSPMV_OK;
+    });
+}
+
+static int validate_csr(const clspmv_csr_matrix* m) {
+    const int rows = m->matinfo.height, cols = m->matinfo.width;
+    if (rows < 0 || m->csr_row_ptr[0] != 0) return CLSPMV_E_INVALID;
+    for (int r = 0; r < rows; ++r)
+        if (m->csr_row_ptr[r + 1] < m->csr_row_ptr[r]) return CLSPMV_E_INVALID;
+    if (!cols_ok(m->csr_col_id, (size_t)m->csr_row_ptr[rows], 0, cols)) {
+        set_error("CSR column index out of range");
+        return CLSPMV_E_INVALID;
+    }
+    return CLSPMV_OK;
+}
+
+static int upload_csr(clspmv_ctx_t ctx, clspmv_part_t p, const clspmv_csr_matrix* m) {
+    int rc = validate_csr(m);
+    if (rc) return rc;
+    const int rows = m->matinfo.height;
+    const size_t nnz = (size_t)m->csr_row_ptr[rows];
+    p->csr.row_ptr = (const int*)p->add(ctx, m->csr_row_ptr, ((size_t)rows + 1) * 4, &rc);
+    if (rc) return rc;
+    p->csr.col = (const int*)p->add(ctx, m->csr_col_id, nnz * 4, &rc);
+    if (rc) return rc;
+    p->csr.val = (const float*)p->add(ctx, m->csr_data, nnz * 4, &rc);
+    if (rc) return rc;
+    p->csr.rows = rows;
+    return CLSPMV_OK;
+}
+
+int clspmv_part_csr_scalar(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        int rc = upload_csr(ctx, p, m);
+        if (rc) return rc;
+        fill_info(p, CLSPMV_FMT_CSR_SCALAR, 0, m->matinfo.height, m->matinfo.width, m->matinfo.nnz, "csr_scalar", 1);
+        (void)method;
+        return CLSPMV_OK;
+    });
+}
+
+// method 0: L lanes per row (L from the mean row length);
+// method 1: adaptive row blocks (stream / vector / long-row split).
+int clspmv_part_csr_vector(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        int rc = upload_csr(ctx, p, m);
+        if (rc) return rc;
+        const int rows = m->matinfo.height;
+        const int* rp = m->csr_row_ptr;
+        const long long nnz = rp[rows];
+        if (method == 0) {
+            double mean = rows ? (double)nnz / rows : 0.0;
+            p->csr_lanes = std::min(64, std::max(2, pow2_floor((int)(mean / 2.0 + 0.5))));
+            char name[32];
+            std::snprintf(name, sizeof(name), "csr_vector_L%d", p->csr_lanes);
+            fill_info(p, CLSPMV_FMT_CSR_VECTOR, 0, rows, m->matinfo.width, m->matinfo.nnz, name, 1);
+            return CLSPMV_OK;
+        }
+        std::vector<int4> blocks;
+        std::vector<int> long_rows, long_first{0};
+        int slots = 0;
+        int r = 0;
+        while (r < rows) {
+            const int len = rp[r + 1] - rp[r];
+            if (len > kCsrStreamMaxRowLen) {
+                if (len <= kCsrLongChunk) {
+                    blocks.push_back(make_int4(r, r + 1, rp[r], rp[r + 1]));
+                } else {
+                    long_rows.push_back(r);
+                    for (int n0 = rp[r]; n0 < rp[r + 1]; n0 += kCsrLongChunk) {
+                        int n1 = std::min(rp[r + 1], n0 + kCsrLongChunk);
+                        blocks.push_back(make_int4(r, -(slots + 1), n0, n1));
+                        ++slots;
+                    }
+                    long_first.push_back(slots);
+                }
+                ++r;
+                continue;
+            }
+            const int r0 = r;
+            int n = 0;
+            while (r < rows && r - r0 < 64) {
+                const int l = rp[r + 1] - rp[r];
+                if (l > kCsrStreamMaxRowLen || n + l > kCsrStreamMaxNnz) break;
+                n += l;
+                ++r;
+            }
+            blocks.push_back(make_int4(r0, r, rp[r0], rp[r]));
+        }
+        p->csra.csr = p->csr;
+        p->csra.nblocks = (int)blocks.size();
+        p->csra.blocks = (const int4*)p->add(ctx, blocks.data(), blocks.size() * sizeof(int4), &rc);
+        if (rc) return rc;
+        p->csra.n_long = (int)long_rows.size();
+        p->csra.carry = (double*)p->scratch((size_t)std::max(1, slots) * sizeof(double), &rc);
+        if (rc) return rc;
+        p->csra.long_rows = (const int*)p->add(ctx, long_rows.data(), long_rows.size() * 4, &rc);
+        if (rc) return rc;
+        p->csra.long_first = (const int*)p->add(ctx, long_first.data(), long_first.size() * 4, &rc);
+        if (rc) return rc;
+        fill_info(p, CLSPMV_FMT_CSR_VECTOR, 1, rows, m->matinfo.width, m->matinfo.nnz, "csr_adaptive",
+                  p->csra.n_long ? 2 : 1);
+        // the carry buffer is scratch, not format storage
+        p->info.stored_bytes -= (long long)std::max(1, slots) * 8;
+        p->info.alg_bytes -= (long long)std::max(1, slots) * 8;
+        return CLSPMV_OK;
+    });
+}
+
+int clspmv_part_coo(clspmv_ctx_t ctx, const clspmv_coo_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        const int rows = m->matinfo.height, cols = m->matinfo.width, nnz = m->matinfo.nnz;
+        if (nnz < 0) return CLSPMV_E_INVALID;
+        for (int k = 0; k < nnz; ++k) {
+            if (m->coo_row_id[k] < 0 || m->coo_row_id[k] >= rows || m->coo_col_id[k] < 0 || m->coo_col_id[k] >= cols) {
+                set_error("COO index out of range at %d", k);
+                return CLSPMV_E_INVALID;
+            }
+            if (k && m->coo_row_id[k] < m->coo_row_id[k - 1]) {
+                set_error("COO entries must be sorted by row");
+                return CLSPMV_E_INVALID;
+            }
+        }
+        const int chunks = (nnz + kCooChunk - 1) / kCooChunk;
+        const size_t padded = (size_t)chunks * kCooChunk;
+        std::vector<int> rr(padded), cc(padded);
+        std::vector<float> vv(padded, 0.0f);
+        if (nnz) {
+            std::memcpy(rr.data(), m->coo_row_id, (size_t)nnz * 4);
+            std::memcpy(cc.data(), m->coo_col_id, (size_t)nnz * 4);
+            std::memcpy(vv.data(), m->coo_data, (size_t)nnz * 4);
+            for (size_t k = (size_t)nnz; k < padded; ++k) {
+                rr[k] = m->coo_row_id[nnz - 1];
+                cc[k] = m->coo_col_id[nnz - 1];
+            }
+        }
+        int rc = 0;
+        p->coo.row = (const int*)p->add(ctx, rr.data(), padded * 4, &rc);
+        if (rc) return rc;
+        p->coo.col = (const int*)p->add(ctx, cc.data(), padded * 4, &rc);
+        if (rc) return rc;
+        p->coo.val = (const float*)p->add(ctx, vv.data(), padded * 4, &rc);
+        if (rc) return rc;
+        p->coo.nnz_padded = (int)padded;
+        p->coo.chunks = chunks;
+        p->coo.rows = rows;
+        const size_t ncarry = (size_t)std::max(1, 2 * chunks);
+        p->coo.carry_row = (int*)p->scratch(ncarry * 4, &rc);
+        if (rc) return rc;
+        p->coo.carry_val = (double*)p->scratch(ncarry * 8, &rc);
+        if (rc) return rc;
+        fill_info(p, CLSPMV_FMT_COO, 0, rows, cols, nnz, "coo_segment", 2);
+        p->info.stored_bytes -= (long long)ncarry * 12;
+        p->info.alg_bytes -= (long long)ncarry * 12;
+        (void)method;
+        return CLSPMV_OK;
+    });
+}
+
+int clspmv_part_dia(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        const int rows = m->matinfo.height, cols = m->matinfo.width, L = m->dia_length_aligned, D = m->dia_num;
+        if (D < 0 || L < rows || rows < 0) return CLSPMV_E_INVALID;
+        int rc = 0;
+        p->dia.offsets = (const int*)p->add(ctx, m->dia_offsets, (size_t)D * 4, &rc);
+        if (rc) return rc;
+        p->dia.data = (const float*)p->add(ctx, m->dia_data, (size_t)D * L * 4, &rc);
+        if (rc) return rc;
+        p->dia.L = L;
+        p->dia.ndiag = D;
+        p->dia.rows = rows;
+        p->dia.cols = cols;
+        int meth = (method == 1 || method == 3) ? 1 : 0;
+        if (L % 4) meth = 0;
+        fill_info(p, CLSPMV_FMT_DIA, meth, rows, cols, m->matinfo.nnz, meth ? "dia_row4" : "dia_row", 1);
+        return CLSPMV_OK;
+    });
+}
+
+int clspmv_part_bdia(clspmv_ctx_t ctx, const clspmv_bdia_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        const int rows = m->matinfo.height, cols = m->matinfo.width, L = m->bdia_length_aligned;
+        const int nb = m->bdia_band_num;
+        if (nb < 0 || L < rows || m->bdia_bptr[0] != 0) return CLSPMV_E_INVALID;
+        int maxw = 0;
+        for (int b = 0; b < nb; ++b) {
+            int w = m->bdia_bptr[b + 1] - m->bdia_bptr[b];
+            if (w < 0) return CLSPMV_E_INVALID;
+            maxw = std::max(maxw, w);
+        }
+        const int D = m->bdia_bptr[nb];
+        int rc = 0;
+        p->bdia.bptr = (const int*)p->add(ctx, m->bdia_bptr, ((size_t)nb + 1) * 4, &rc);
+        if (rc) return rc;
+        p->bdia.offsets = (const int*)p->add(ctx, m->bdia_offsets, (size_t)nb * 4, &rc);
+        if (rc) return rc;
+        p->bdia.data = (const float*)p->add(ctx, m->bdia_data, (size_t)D * L * 4, &rc);
+        if (rc) return rc;
+        p->bdia.L = L;
+        p->bdia.nbands = nb;
+        p->bdia.rows = rows;
+        p->bdia.cols = cols;
+        // methods 1 (gpu_bdia) and 4 (gpu_bdia_v4) stage x in LDS
+        int meth = (method == 1 || method == 2 || method == 4) ? 1 : 0;
+        if (maxw > 257) meth = 0;
+        fill_info(p, CLSPMV_FMT_BDIA, meth, rows, cols, m->matinfo.nnz, meth ? "bdia_lds" : "bdia_row", 1);
+        return CLSPMV_OK;
+    });
+}
+
+static bool block_shape_ok(unsigned bw, unsigned bh) {
+    return (bw == 4 || bw == 8) && (bh == 1 || bh == 2 || bh == 4 || bh == 8);
+}
+
+int clspmv_part_bell(clspmv_ctx_t ctx, const clspmv_b4ell_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        const unsigned bw = m->b4ell_bwidth, bh = m->b4ell_bheight;
+        if (!block_shape_ok(bw, bh)) return CLSPMV_E_UNSUPPORTED;
+        const int rows = m->matinfo.height, cols = m->matinfo.width;
+        const int brows = m->b4ell_row_num, Hc = m->b4ell_height_aligned, F = m->b4ell_float4_aligned;
+        const int slots = m->b4ell_block_num;
+        if (Hc < brows || F < 4 * brows || (F % 4) || brows != (rows + (int)bh - 1) / (int)bh) return CLSPMV_E_INVALID;
+        const int c4max = (cols + 3) / 4;
+        for (int j = 0; j < slots; ++j)
+            if (!cols_ok(m->b4ell_col_id + (size_t)j * Hc, (size_t)brows, 0, std::max(1, c4max))) {
+                set_error("BELL block column out of range");
+                return CLSPMV_E_INVALID;
+            }
+        int rc = 0;
+        p->bell.col = (const int*)p->add(ctx, m->b4ell_col_id, (size_t)Hc * slots * 4, &rc);
+        if (rc) return rc;
+        p->bell.data = (const float*)p->add(ctx, m->b4ell_data, (size_t)F * bh * (bw / 4) * slots * 4, &rc);
+        if (rc) return rc;
+        p->bell = DevBell{p->bell.col, p->bell.data, Hc, F, slots, brows, (int)bh, (int)bw / 4, rows, cols};
+        char name[32];
+        std::snprintf(name, sizeof(name), "bell%u%u", bh, bw);
+        fill_info(p, CLSPMV_FMT_BELL, method & 1, rows, cols, m->matinfo.nnz, name, 1);
+        return CLSPMV_OK;
+    });
+}
+
+int clspmv_part_sbell(clspmv_ctx_t ctx, const clspmv_sbell_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        const unsigned bw = m->sbell_bwidth, bh = m->sbell_bheight;
+        if (!block_shape_ok(bw, bh)) return CLSPMV_E_UNSUPPORTED;
+        const int rows = m->matinfo.height, cols = m->matinfo.width;
+        const int S = (int)m->sbell_slice_height, ns = m->sbell_slice_num, brows = m->sbell_row_num;
+        if (S <= 0 || brows != (rows + (int)bh - 1) / (int)bh || ns != (brows + S - 1) / S) return CLSPMV_E_INVALID;
+        if (m->sbell_slice_ptr[0] != 0) return CLSPMV_E_INVALID;
+        const int c4max = std::max(1, (cols + 3) / 4);
+        for (int s = 0; s < ns; ++s) {
+            int a = m->sbell_slice_ptr[s], b = m->sbell_slice_ptr[s + 1];
+            if (b < a || (b - a) % S) return CLSPMV_E_INVALID;
+            int w = (b - a) / S;
+            for (int r = s * S; r < std::min(brows, s * S + S); ++r)
+                for (int j = 0; j < w; ++j) {
+                    int c = m->sbell_col_id[(size_t)a + (size_t)j * S + (r - s * S)];
+                    if (c < 0 || c >= c4max) {
+                        set_error("SBELL block column out of range");
+                        return CLSPMV_E_INVALID;
+                    }
+                }
+        }
+        const size_t total = (size_t)m->sbell_slice_ptr[ns];
+        int rc = 0;
+        p->sbell.slice_ptr = (const int*)p->add(ctx, m->sbell_slice_ptr, ((size_t)ns + 1) * 4, &rc);
+        if (rc) return rc;
+        p->sbell.col = (const int*)p->add(ctx, m->sbell_col_id, total * 4, &rc);
+        if (rc) return rc;
+        p->sbell.data = (const float*)p->add(ctx, m->sbell_data, total * bw * bh * 4, &rc);
+        if (rc) return rc;
+        p->sbell.S = S;
+        p->sbell.brows = brows;
+        p->sbell.bh = (int)bh;
+        p->sbell.w4n = (int)bw / 4;
+        p->sbell.rows = rows;
+        p->sbell.cols = cols;
+        char name[32];
+        std::snprintf(name, sizeof(name), "sbell%u%u", bh, bw);
+        fill_info(p, CLSPMV_FMT_SBELL, 0, rows, cols, m->matinfo.nnz, name, 1);
+        (void)method;
+        return CLSPMV_OK;
+    });
+}
+
+int clspmv_part_bcsr(clspmv_ctx_t ctx, const clspmv_b4csr_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        const unsigned bw = m->b4csr_bwidth, bh = m->b4csr_bheight;
+        if (!block_shape_ok(bw, bh)) return CLSPMV_E_UNSUPPORTED;
+        const int rows = m->matinfo.height, cols = m->matinfo.width;
+        const int brows = m->b4csr_row_num, nb = m->b4csr_block_num, N = m->b4csr_aligned_size;
+        if (brows != (rows + (int)bh - 1) / (int)bh || N < 4 * nb || (N % 4) || m->b4csr_row_ptr[0] != 0 ||
+            m->b4csr_row_ptr[brows] != nb)
+            return CLSPMV_E_INVALID;
+        for (int r = 0; r < brows; ++r)
+            if (m->b4csr_row_ptr[r + 1] < m->b4csr_row_ptr[r]) return CLSPMV_E_INVALID;
+        if (!cols_ok(m->b4csr_col_id, (size_t)nb, 0, std::max(1, (cols + 3) / 4))) {
+            set_error("BCSR block column out of range");
+            return CLSPMV_E_INVALID;
+        }
+        int rc = 0;
+        p->bcsr.row_ptr = (const int*)p->add(ctx, m->b4csr_row_ptr, ((size_t)brows + 1) * 4, &rc);
+        if (rc) return rc;
+        p->bcsr.col = (const int*)p->add(ctx, m->b4csr_col_id, (size_t)nb * 4, &rc);
+        if (rc) return rc;
+        p->bcsr.data = (const float*)p->add(ctx, m->b4csr_data, (size_t)N * bh * (bw / 4) * 4, &rc);
+        if (rc) return rc;
+        double mean = brows ? (double)nb / brows : 0.0;
+        int lanes = std::min(16, std::max(2, pow2_floor((int)(mean + 0.5))));
+        p->bcsr = DevBcsr{p->bcsr.row_ptr, p->bcsr.col, p->bcsr.data, N / 4, brows, (int)bh, (int)bw / 4, rows, cols, lanes};
+        char name[32];
+        std::snprintf(name, sizeof(name), "bcsr%u%u_L%d", bh, bw, lanes);
+        fill_info(p, CLSPMV_FMT_BCSR, 0, rows, cols, m->matinfo.nnz, name, 1);
+        (void)method;
+        return CLSPMV_OK;
+    });
+}
+
+int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        const unsigned bw = m->bsr_bwidth, bh = m->bsr_bheight;
+        if (bw != bh || bw < 2 || bw > 6) return CLSPMV_E_UNSUPPORTED;
+        const int rows = m->matinfo.height, cols = m->matinfo.width;
+        const int brows = m->bsr_row_num, nb = m->bsr_block_num, stride = m->bsr_block_stride;
+        if (brows != (rows + (int)bh - 1) / (int)bh || stride < nb || m->bsr_row_ptr[0] != 0 ||
+            m->bsr_row_ptr[brows] != nb)
+            return CLSPMV_E_INVALID;
+        for (int r = 0; r < brows; ++r)
+            if (m->bsr_row_ptr[r + 1] < m->bsr_row_ptr[r]) return CLSPMV_E_INVALID;
+        if (!cols_ok(m->bsr_col_id, (size_t)nb, 0, std::max(1, cols))) {
+            set_error("BSR block column out of range");
+            return CLSPMV_E_INVALID;
+        }
+        int rc = 0;
+        p->bsr.row_ptr = (const int*)p->add(ctx, m->bsr_row_ptr, ((size_t)brows + 1) * 4, &rc);
+        if (rc) return rc;
+        p->bsr.col = (const int*)p->add(ctx, m->bsr_col_id, (size_t)nb * 4, &rc);
+        if (rc) return rc;
+        p->bsr.data = (const float*)p->add(ctx, m->bsr_data, (size_t)stride * bw * bh * 4, &rc);
+        if (rc) return rc;
+        double mean = brows ? (double)nb / brows : 0.0;
+        int lanes = (method >= 2 && method <= 16) ? pow2_floor(method) : std::min(16, std::max(2, pow2_floor((int)(mean + 0.5))));
+        p->bsr = DevBsr{p->bsr.row_ptr, p->bsr.col, p->bsr.data, stride, brows, (int)bh, (int)bw, rows, cols, lanes};
+        char name[32];
+        std::snprintf(name, sizeof(name), "bsr%u%u_L%d", bh, bw, lanes);
+        fill_info(p, CLSPMV_FMT_BSR, lanes, rows, cols, m->matinfo.nnz, name, 1);
+        return CLSPMV_OK;
+    });
+}
+
+int clspmv_part_launch(clspmv_part_t p, const float* x, float* y, hipStream_t st) {
+    switch (p->info.format) {
+        case CLSPMV_FMT_ELL: return launch_ell(p->ell, p->info.method, x, y, st);
+        case CLSPMV_FMT_SELL: return launch_sell(p->sell, x, y, st);
+        case CLSPMV_FMT_CSR_SCALAR: return launch_csr_scalar(p->csr, x, y, st);
+        case CLSPMV_FMT_CSR_VECTOR:
+            return p->info.method == 0 ? launch_csr_vector(p->csr, p->csr_lanes, x, y, st)
+                                       : launch_csr_adaptive(p->csra, x, y, st);
+        case CLSPMV_FMT_COO: return launch_coo(p->coo, x, y, st);
+        case CLSPMV_FMT_DIA: return launch_dia(p->dia, p->info.method, x, y, st);
+        case CLSPMV_FMT_BDIA: return launch_bdia(p->bdia, p->info.method, x, y, st);
+        case CLSPMV_FMT_BELL: return launch_bell(p->bell, x, y, st);
+        case CLSPMV_FMT_SBELL: return launch_sbell(p->sbell, x, y, st);
+        case CLSPMV_FMT_BCSR: return launch_bcsr(p->bcsr, x, y, st);
+        case CLSPMV_FMT_BSR: return launch_bsr(p->bsr, x, y, st);
+        default: return (int)hipErrorInvalidValue;
+    }
+}
+
+int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes) {
+    if (!p || !d_x || !d_y || ntimes < 0) return CLSPMV_E_INVALID;
+    if ((reinterpret_cast<uintptr_t>(d_x) | reinterpret_cast<uintptr_t>(d_y)) & 15) {
+        set_error("x and y must be 16-byte aligned device pointers");
+        return CLSPMV_E_INVALID;
+    }
+    hipStream_t st = p->ctx->stream();
+    for (int i = 0; i < ntimes; ++i) {
+        int e = clspmv_part_launch(p, d_x, d_y, st);
+        if (e) return hip_status((hipError_t)e, "kernel launch");
+    }
+    return CLSPMV_OK;
+}
+
+int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info) {
+    if (!p || !info) return CLSPMV_E_INVALID;
+    *info = p->info;
+    return CLSPMV_OK;
+}
+
+int clspmv_part_free(clspmv_part_t p) {
+    if (!p) return CLSPMV_E_INVALID;
+    (void)hipStreamSynchronize(p->ctx->stream());
+    delete p;
+    return CLSPMV_OK;
+}
+
+// ---------------------------------------------------------------- probes
+
+int clspmv_probe_copy(clspmv_ctx_t ctx, const void* d_src, void* d_dst, size_t bytes, int ntimes, float* gbps) {
+    if (!ctx || !gbps || (bytes % 16) || ntimes <= 0) return CLSPMV_E_INVALID;
+    hipStream_t st = ctx->stream();
+    int e = launch_copy_probe((const float4*)d_src, (float4*)d_dst, bytes / 16, st);
+    if (e) return hip_status((hipError_t)e, "copy probe");
+    (void)hipEventRecord(ctx->ev0, st);
+    for (int i = 0; i < ntimes; ++i) launch_copy_probe((const float4*)d_src, (float4*)d_dst, bytes / 16, st);
+    (void)hipEventRecord(ctx->ev1, st);
+    hipError_t he = hipEventSynchronize(ctx->ev1);
+    if (he != hipSuccess) return hip_status(he, "copy probe sync");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    *gbps = (float)(2.0 * (double)bytes * ntimes / (ms * 1e-3) / 1e9);
+    return CLSPMV_OK;
+}
+
+int clspmv_probe_launch(clspmv_ctx_t ctx, int ntimes, float* usec) {
+    if (!ctx || !usec || ntimes <= 0) return CLSPMV_E_INVALID;
+    hipStream_t st = ctx->stream();
+    launch_empty(st);
+    (void)hipEventRecord(ctx->ev0, st);
+    for (int i = 0; i < ntimes; ++i) launch_empty(st);
+    (void)hipEventRecord(ctx->ev1, st);
+    hipError_t he = hipEventSynchronize(ctx->ev1);
+    if (he != hipSuccess) return hip_status(he, "launch probe sync");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    *usec = ms * 1000.f / ntimes;
+    return CLSPMV_OK;
+}
+
+}  // extern "C"

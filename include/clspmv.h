@@ -1,0 +1,326 @@
+/*
+ * clspmv.h -- C ABI of the MI355X-native clSpMV engine (libclspmv.so).
+ *
+ * This header is the drop-in boundary.  The reference (clSpMV, OpenCL 1.1)
+ * exposes C++ templates instantiated for <int,float> plus an OpenCL runtime
+ * shim; here every entry point is extern "C" with plain pointers and sizes.
+ * Each declaration cites the reference interface it replaces
+ * (paths relative to the reference tree, A/ = clSpMV_AMD/).
+ *
+ * Semantics kept from the reference:
+ *   - every SpMV computes y <- A*x + y (kernels read the old y first,
+ *     A/kernels/spmv_dia.cl:15, A/src/eval.cpp:40-42);
+ *   - struct layouts and field names of A/include/matrix_storage.h:15-365
+ *     and A/include/analyze.h:105-144 (int32 indices, float values);
+ *   - conversions return the reference's bool as CLSPMV_OK / an error code
+ *     and allocate with malloc; the caller releases with clspmv_free_<fmt>.
+ * Differences (documented, never silent):
+ *   - no exit(1)/assert: every failure is an integer status code;
+ *   - OpenCL context/queue/cl_mem become clspmv_ctx_t (device + HIP stream)
+ *     and plain device pointers (void*);
+ *   - the texture (image2d) kernel variants do not exist on CDNA4; the
+ *     use_tex flag is accepted and ignored.
+ *
+ * Threading: one host thread per context; calls on one context are not
+ * thread-safe.  All GPU work of a context is issued on its one stream in
+ * order, so Cocktail parts that accumulate into the same y are serialised
+ * exactly as on the reference's in-order queue.
+ */
+#ifndef CLSPMV_H_
+#define CLSPMV_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes (replace CHECKERROR/exit(1), A/include/oclcommon.h:10-38,
+ * and the bool returns of A/include/matrix_storage.h conversions).     */
+#define CLSPMV_OK              0
+#define CLSPMV_E_INVALID      -1  /* bad argument / shape                */
+#define CLSPMV_E_TOO_LARGE    -2  /* MAX_MEM_OBJ refusal (matrix_storage.h:1863,2141,2350) */
+#define CLSPMV_E_PADDING      -3  /* DIA padding >= 2*nnz (matrix_storage.h:972-979) */
+#define CLSPMV_E_HIP          -4  /* HIP runtime error                   */
+#define CLSPMV_E_NOMEM        -5  /* host or device allocation failed    */
+#define CLSPMV_E_IO           -6  /* file cannot be opened / parsed      */
+#define CLSPMV_E_SORT         -7  /* sort_coo exceeded MAX_LEVELS (matrix_storage.h:774) */
+#define CLSPMV_E_UNSUPPORTED  -8  /* shape/method not provided           */
+#define CLSPMV_E_NODEVICE     -9  /* no HIP device visible               */
+
+/* Reference compile-time constants that shape the data layouts
+ * (A/include/constant.h:8-53). */
+#define CLSPMV_GPU_ALIGNMENT   32
+#define CLSPMV_COO_ALIGNMENT   32
+#define CLSPMV_WARPSIZE        32
+#define CLSPMV_MAX_MEM_OBJ     268435456
+#define CLSPMV_MAX_DIA_NUM     128
+#define CLSPMV_MAX_BAND_NUM    31
+#define CLSPMV_MAX_LEVELS      1000
+
+/* Format ids, numbered like spmv_all's methods (A/src/spmv_all.cpp:196-250). */
+#define CLSPMV_FMT_CSR_SCALAR  1
+#define CLSPMV_FMT_CSR_VECTOR  2
+#define CLSPMV_FMT_BDIA        3
+#define CLSPMV_FMT_DIA         4
+#define CLSPMV_FMT_ELL         5
+#define CLSPMV_FMT_COO         6
+#define CLSPMV_FMT_BELL        7
+#define CLSPMV_FMT_BCSR        8
+#define CLSPMV_FMT_SELL        9
+#define CLSPMV_FMT_SBELL      10
+#define CLSPMV_FMT_BSR        11  /* native r x c BCSR (e.g. 3x3); no reference counterpart */
+
+/* ------------------------------------------------------------------ */
+/* Data model: A/include/matrix_storage.h:15-365 (<int,int,float>).    */
+typedef struct { int width; int height; int nnz; } clspmv_matinfo;
+
+typedef struct {                 /* coo_matrix :141-152 */
+    clspmv_matinfo matinfo;
+    int*   coo_row_id;
+    int*   coo_col_id;
+    float* coo_data;
+} clspmv_coo_matrix;
+
+typedef struct {                 /* csr_matrix :155-166 */
+    clspmv_matinfo matinfo;
+    int*   csr_row_ptr;
+    int*   csr_col_id;
+    float* csr_data;
+} clspmv_csr_matrix;
+
+typedef struct {                 /* dia_matrix :30-47 */
+    clspmv_matinfo matinfo;
+    int    dia_num;
+    int    dia_length;
+    int    dia_length_aligned;
+    int*   dia_offsets;
+    float* dia_data;
+} clspmv_dia_matrix;
+
+typedef struct {                 /* bdia_matrix :72-94 */
+    clspmv_matinfo matinfo;
+    int    bdia_band_num;
+    int    bdia_length;
+    int    bdia_length_aligned;
+    int*   bdia_offsets;
+    int*   bdia_bptr;
+    float* bdia_data;
+} clspmv_bdia_matrix;
+
+typedef struct {                 /* ell_matrix :96-111 */
+    clspmv_matinfo matinfo;
+    int    ell_num;
+    int    ell_height_aligned;
+    int*   ell_col_id;
+    float* ell_data;
+} clspmv_ell_matrix;
+
+typedef struct {                 /* sell_matrix :122-137 */
+    clspmv_matinfo matinfo;
+    unsigned int sell_slice_height;
+    int    sell_slice_num;
+    int*   sell_slice_ptr;
+    int*   sell_col_id;
+    float* sell_data;
+} clspmv_sell_matrix;
+
+typedef struct {                 /* b4csr_matrix :266-287 (GPU BCSR) */
+    clspmv_matinfo matinfo;
+    unsigned int b4csr_bwidth;
+    unsigned int b4csr_bheight;
+    int    b4csr_row_num;
+    int    b4csr_aligned_size;
+    int    b4csr_block_num;
+    int*   b4csr_row_ptr;
+    int*   b4csr_col_id;
+    float* b4csr_data;
+} clspmv_b4csr_matrix;
+
+typedef struct {                 /* b4ell_matrix :311-332 (GPU BELL) */
+    clspmv_matinfo matinfo;
+    unsigned int b4ell_bwidth;
+    unsigned int b4ell_bheight;
+    int    b4ell_row_num;
+    int    b4ell_block_num;
+    int    b4ell_height_aligned;
+    int    b4ell_float4_aligned;
+    int*   b4ell_col_id;
+    float* b4ell_data;
+} clspmv_b4ell_matrix;
+
+typedef struct {                 /* sbell_matrix :344-365 */
+    clspmv_matinfo matinfo;
+    unsigned int sbell_bwidth;
+    unsigned int sbell_bheight;
+    unsigned int sbell_slice_height;
+    int    sbell_slice_num;
+    int    sbell_row_num;
+    int*   sbell_slice_ptr;
+    int*   sbell_col_id;
+    float* sbell_data;
+} clspmv_sbell_matrix;
+
+/* Native r x c block CSR (no reference counterpart; the reference only has
+ * {1,2,4,8}x{4,8} shapes, A/src/analyze.cpp:545-546).  Blocks of one block
+ * row are contiguous; values are stored element-major:
+ *   bsr_data[e * bsr_block_stride + i],  e = h*bwidth + w,  i = block id,
+ * bsr_block_stride = align(block_num, 64), so lanes that read consecutive
+ * blocks issue one coalesced stream per element slot.                  */
+typedef struct {
+    clspmv_matinfo matinfo;
+    unsigned int bsr_bwidth;
+    unsigned int bsr_bheight;
+    int    bsr_row_num;          /* ceil(height / bheight) */
+    int    bsr_block_num;
+    int    bsr_block_stride;
+    int*   bsr_row_ptr;          /* size bsr_row_num + 1 */
+    int*   bsr_col_id;           /* first column of each block (element units) */
+    float* bsr_data;             /* size bsr_block_stride * bwidth * bheight */
+} clspmv_bsr_matrix;
+
+/* ------------------------------------------------------------------ */
+/* Host data model entry points.                                        */
+
+/* init_xxx / free_xxx (matrix_storage.h:370-652): zero the struct or free
+ * its arrays (pointers are reset to NULL). */
+void clspmv_init_coo(clspmv_coo_matrix* m);
+void clspmv_free_coo(clspmv_coo_matrix* m);
+void clspmv_free_csr(clspmv_csr_matrix* m);
+void clspmv_free_dia(clspmv_dia_matrix* m);
+void clspmv_free_bdia(clspmv_bdia_matrix* m);
+void clspmv_free_ell(clspmv_ell_matrix* m);
+void clspmv_free_sell(clspmv_sell_matrix* m);
+void clspmv_free_b4csr(clspmv_b4csr_matrix* m);
+void clspmv_free_b4ell(clspmv_b4ell_matrix* m);
+void clspmv_free_sbell(clspmv_sbell_matrix* m);
+void clspmv_free_bsr(clspmv_bsr_matrix* m);
+
+/* Allocate a COO of the given shape (arrays malloc'd, uninitialised). */
+int clspmv_alloc_coo(clspmv_coo_matrix* m, int width, int height, int nnz);
+
+/* if_sorted_coo (matrix_storage.h:731-741): 1 if row-major sorted. */
+int clspmv_if_sorted_coo(const clspmv_coo_matrix* m);
+/* sort_coo (matrix_storage.h:744-815): in-place, same element order as the
+ * reference for equal keys; CLSPMV_E_SORT past MAX_LEVELS. */
+int clspmv_sort_coo(clspmv_coo_matrix* m);
+/* aligned_length / findPaddedSize (matrix_storage.h:817, util.cpp:17). */
+int clspmv_aligned_length(int length, int alignment);
+
+/* Conversions.  Each sorts an unsorted source in place first, as the
+ * reference does.                                                      */
+int clspmv_coo2csr(clspmv_coo_matrix* src, clspmv_csr_matrix* dst);                          /* :1268 */
+int clspmv_coo2dia(clspmv_coo_matrix* src, clspmv_dia_matrix* dst, int alignment);          /* :936  */
+int clspmv_coo2bdia(clspmv_coo_matrix* src, clspmv_bdia_matrix* dst, int alignment);        /* :1184 */
+int clspmv_coo2ell(clspmv_coo_matrix* src, clspmv_ell_matrix* dst, int alignment, int ellnum); /* :1340 */
+int clspmv_coo2sell(clspmv_coo_matrix* src, clspmv_sell_matrix* dst, unsigned int slice_height); /* :1397 */
+int clspmv_coo2b4csr(clspmv_coo_matrix* src, clspmv_b4csr_matrix* dst,
+                     unsigned int bwidth, unsigned int bheight, int alignment);             /* :1740 */
+int clspmv_coo2b4ell(clspmv_coo_matrix* src, clspmv_b4ell_matrix* dst,
+                     unsigned int bwidth, unsigned int bheight, int alignment, int ellnum); /* :2009 */
+int clspmv_coo2sbell(clspmv_coo_matrix* src, clspmv_sbell_matrix* dst,
+                     unsigned int bwidth, unsigned int bheight, unsigned int slice_height); /* :2202 */
+/* Native r x c BCSR (build extension, any 1<=r,c<=8). */
+int clspmv_coo2bsr(clspmv_coo_matrix* src, clspmv_bsr_matrix* dst,
+                   unsigned int bwidth, unsigned int bheight);
+/* pad_csr (A/src/util.cpp:31-72). */
+int clspmv_pad_csr(const clspmv_csr_matrix* src, clspmv_csr_matrix* dst, int alignment);
+
+/* ReadMMF (A/src/fileio.cpp:10-109), same quirks: the 4th header token
+ * "real" selects real values, anything else reads 1.0 without consuming a
+ * value token; "symmetric" mirrors off-diagonal entries; duplicates kept. */
+int clspmv_read_mmf(const char* filename, clspmv_coo_matrix* m);
+/* Matrix Market writer (real general, 1-based, %.9g values). */
+int clspmv_write_mmf(const char* filename, const clspmv_coo_matrix* m);
+/* write_coo (matrix_storage.h:656-671): width,height,nnz,rows,cols,data. */
+int clspmv_write_coo_bin(const char* filename, const clspmv_coo_matrix* m);
+/* Working replacement of the broken read_coo (matrix_storage.h:673-696). */
+int clspmv_read_coo_bin(const char* filename, clspmv_coo_matrix* m);
+
+/* ------------------------------------------------------------------ */
+/* Seeded synthetic generators (SplitMix64-seeded xoshiro256**; no
+ * duplicate (row, col); output row-major sorted).                      */
+int clspmv_gen_laplacian2d(clspmv_coo_matrix* m, int nx, int ny);
+int clspmv_gen_stencil27(clspmv_coo_matrix* m, int n, uint64_t seed);
+int clspmv_gen_uniform_rows(clspmv_coo_matrix* m, int rows, int per_row, uint64_t seed);
+int clspmv_gen_zipf_rows(clspmv_coo_matrix* m, int rows, double exponent, double mean,
+                         int max_len, uint64_t seed);
+int clspmv_gen_block_rows(clspmv_coo_matrix* m, int block_rows, int bdim, int blocks_per_row,
+                          int band, uint64_t seed);
+int clspmv_gen_cocktail_composite(clspmv_coo_matrix* m, int rows, uint64_t seed);
+/* x_i ~ U[-1,1] from seed. */
+int clspmv_gen_vector(float* x, int n, uint64_t seed);
+
+/* ------------------------------------------------------------------ */
+/* HIP runtime: replaces initialization()/freeObjects()
+ * (A/src/oclcommon.cpp:259-422, A/include/oclcommon.h:45-46).          */
+typedef struct clspmv_ctx_s* clspmv_ctx_t;
+
+int clspmv_init(int device, clspmv_ctx_t* ctx);
+int clspmv_finalize(clspmv_ctx_t ctx);
+/* Use an external hipStream_t (NULL = the context's own stream). */
+int clspmv_set_stream(clspmv_ctx_t ctx, void* stream);
+int clspmv_synchronize(clspmv_ctx_t ctx);
+const char* clspmv_last_error(void);
+
+/* Device buffers (ALLOCATE_GPU_READ, oclcommon.h:20-38). */
+int clspmv_device_alloc(clspmv_ctx_t ctx, size_t bytes, void** dptr);
+int clspmv_device_free(clspmv_ctx_t ctx, void* dptr);
+int clspmv_memcpy_h2d(clspmv_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int clspmv_memcpy_d2h(clspmv_ctx_t ctx, void* dst, const void* src, size_t bytes);
+int clspmv_memcpy_d2d(clspmv_ctx_t ctx, void* dst, const void* src, size_t bytes);
+
+/* HIP-event timing on the context stream (replaces gettimeofday around
+ * clFinish, A/src/util.cpp:10-15). */
+int clspmv_event_elapsed_begin(clspmv_ctx_t ctx);
+int clspmv_event_elapsed_end(clspmv_ctx_t ctx, float* ms);
+
+/* ------------------------------------------------------------------ */
+/* Single-format device parts: the device half of spmv_<fmt>_ocl
+ * (e.g. A/src/spmv_ell.cpp:17-300).  A part owns the uploaded format
+ * arrays and the chosen kernel variant; clspmv_part_spmv enqueues
+ * ntimes * (y += A*x) on the context stream.  d_x must hold
+ * matinfo.width floats, d_y matinfo.height floats (device pointers,
+ * 16-byte aligned).                                                    */
+typedef struct clspmv_part_s* clspmv_part_t;
+
+typedef struct {
+    int    format;         /* CLSPMV_FMT_* */
+    int    method;         /* kernel variant actually used */
+    int    rows, cols;
+    long long nnz;         /* true nonzeros (FLOPs = 2*nnz) */
+    long long stored_bytes;/* bytes of every uploaded format array */
+    long long alg_bytes;   /* stored_bytes + 4*cols + 8*rows (x read, y read+write) */
+    int    launches;       /* kernels per SpMV */
+    char   kernel[64];     /* name of the dominant kernel */
+} clspmv_part_info;
+
+int clspmv_part_csr_scalar(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int method, clspmv_part_t* p);
+int clspmv_part_csr_vector(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int method, clspmv_part_t* p);
+int clspmv_part_dia(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int method, clspmv_part_t* p);
+int clspmv_part_bdia(clspmv_ctx_t ctx, const clspmv_bdia_matrix* m, int method, clspmv_part_t* p);
+int clspmv_part_ell(clspmv_ctx_t ctx, const clspmv_ell_matrix* m, int method, clspmv_part_t* p);
+int clspmv_part_sell(clspmv_ctx_t ctx, const clspmv_sell_matrix* m, int method, clspmv_part_t* p);
+int clspmv_part_coo(clspmv_ctx_t ctx, const clspmv_coo_matrix* m, int method, clspmv_part_t* p);
+int clspmv_part_bell(clspmv_ctx_t ctx, const clspmv_b4ell_matrix* m, int method, clspmv_part_t* p);
+int clspmv_part_sbell(clspmv_ctx_t ctx, const clspmv_sbell_matrix* m, int method, clspmv_part_t* p);
+int clspmv_part_bcsr(clspmv_ctx_t ctx, const clspmv_b4csr_matrix* m, int method, clspmv_part_t* p);
+int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, clspmv_part_t* p);
+
+int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
+int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info);
+int clspmv_part_free(clspmv_part_t p);
+
+/* ------------------------------------------------------------------ */
+/* Bandwidth / launch-overhead probes: bandwidth_test and bench_overhead
+ * (A/src/mem_bandwidth.cpp:14-579, A/src/bench_overhead.cpp:14-184). */
+int clspmv_probe_copy(clspmv_ctx_t ctx, const void* d_src, void* d_dst, size_t bytes, int ntimes, float* gbps);
+int clspmv_probe_launch(clspmv_ctx_t ctx, int ntimes, float* usec_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLSPMV_H_ */

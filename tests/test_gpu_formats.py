@@ -1,0 +1,170 @@
+"""GPU parity of every single-format kernel against the CPU oracle.
+
+Each case uploads one format through the C ABI (clspmv_part_*), runs
+y += A x on the MI355X and compares with the reference's CPU SpMV restated in
+oracle/ (FP32, COO order) using the per-element bound of
+oracle.parity_report: relative error <= 1e-5 against the reference-order
+FP32 result (scale max(|y|, sum_j |a_ij x_j|)), or no worse than that result
+against the FP64 oracle.  y starts non-zero to check the y += semantics.
+"""
+import numpy as np
+import pytest
+
+from clspmv_amd import _lib as L
+from clspmv_amd import matrix as M
+from oracle import oracle as O
+from tests.helpers import gpu_spmv, rand_coo
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    cases = []
+    cases.append(("lap2d_32", lambda: M.gen_laplacian2d(32, 32)))
+    cases.append(("stencil27_12", lambda: M.gen_stencil27(12, 7)))
+    cases.append(("uniform_rows", lambda: M.gen_uniform_rows(3000, 17, 3)))
+    cases.append(("zipf", lambda: M.gen_zipf_rows(5000, 1.8, 16.0, 3000, 11)))
+    cases.append(("blocks3", lambda: M.gen_block_rows(700, 3, 5, 40, 5)))
+    cases.append(("rand_ragged", lambda: rand_coo(333, 301, 4000, 1)))
+    cases.append(("rand_sparse_empty_rows", lambda: rand_coo(517, 517, 90, 2)))
+    cases.append(("one_row", lambda: rand_coo(1, 40, 25, 3)))
+    return cases
+
+
+CASES = _cases()
+IDS = [c[0] for c in CASES]
+
+
+def _x_y(coo, seed=5):
+    x = M.gen_vector(coo.width, seed)
+    y0 = M.gen_vector(coo.height, seed + 1)
+    return x, y0
+
+
+def _check(coo, y, x, y0, what):
+    rep = O.parity_report(y, coo, x, y0)
+    assert rep["ok"], f"{what}: {rep}"
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=IDS)
+@pytest.mark.parametrize("method", [0, 1])
+def test_ell(ctx, name, make, method):
+    coo = make()
+    x, y0 = _x_y(coo)
+    ell = M.coo2ell(coo)
+    y, info = gpu_spmv(ctx, L.FMT_ELL, ell, x, y0, method)
+    _check(coo, y, x, y0, f"ELL m{method} {name}")
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=IDS)
+@pytest.mark.parametrize("S", [32, 64, 128])
+def test_sell(ctx, name, make, S):
+    coo = make()
+    x, y0 = _x_y(coo)
+    y, _ = gpu_spmv(ctx, L.FMT_SELL, M.coo2sell(coo, S), x, y0)
+    _check(coo, y, x, y0, f"SELL S{S} {name}")
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=IDS)
+def test_csr_scalar(ctx, name, make):
+    coo = make()
+    x, y0 = _x_y(coo)
+    y, _ = gpu_spmv(ctx, L.FMT_CSR_SCALAR, M.coo2csr(coo), x, y0)
+    _check(coo, y, x, y0, f"CSR scalar {name}")
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=IDS)
+@pytest.mark.parametrize("method", [0, 1])
+def test_csr_vector(ctx, name, make, method):
+    coo = make()
+    x, y0 = _x_y(coo)
+    y, info = gpu_spmv(ctx, L.FMT_CSR_VECTOR, M.coo2csr(coo), x, y0, method)
+    _check(coo, y, x, y0, f"CSR vector m{method} {name} {info['kernel']}")
+
+
+def test_csr_adaptive_long_rows(ctx):
+    # rows far longer than one 8192-nnz chunk exercise the FP64 carry path
+    rng = np.random.default_rng(9)
+    h, w = 64, 100000
+    lens = np.array([0, 70000, 3, 8192, 8193, 20000] + [int(v) for v in rng.integers(0, 200, h - 6)])
+    rows, cols = [], []
+    for r, l in enumerate(lens):
+        c = np.sort(rng.choice(w, size=int(l), replace=False))
+        rows.append(np.full(int(l), r, np.int32))
+        cols.append(c.astype(np.int32))
+    rows = np.concatenate(rows)
+    cols = np.concatenate(cols)
+    vals = rng.standard_normal(rows.size).astype(np.float32)
+    coo = M.CooMatrix.from_arrays(rows, cols, vals, w, h)
+    x, y0 = _x_y(coo)
+    for method in (0, 1):
+        y, info = gpu_spmv(ctx, L.FMT_CSR_VECTOR, M.coo2csr(coo), x, y0, method)
+        _check(coo, y, x, y0, f"CSR long rows m{method}")
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=IDS)
+def test_coo(ctx, name, make):
+    coo = make()
+    x, y0 = _x_y(coo)
+    y, _ = gpu_spmv(ctx, L.FMT_COO, coo, x, y0)
+    _check(coo, y, x, y0, f"COO {name}")
+
+
+def test_coo_long_rows_and_boundaries(ctx):
+    # rows whose segments end exactly on lane (8) and chunk (512) boundaries
+    lens = [8, 8, 16, 512, 1, 7, 504, 1024, 3, 5000, 0, 0, 9, 511, 513]
+    rows = np.concatenate([np.full(l, r, np.int32) for r, l in enumerate(lens)])
+    cols = np.concatenate([np.arange(l, dtype=np.int32) for l in lens])
+    rng = np.random.default_rng(4)
+    vals = rng.standard_normal(rows.size).astype(np.float32)
+    coo = M.CooMatrix.from_arrays(rows, cols, vals, 6000, len(lens))
+    x, y0 = _x_y(coo)
+    y, _ = gpu_spmv(ctx, L.FMT_COO, coo, x, y0)
+    _check(coo, y, x, y0, "COO boundaries")
+
+
+@pytest.mark.parametrize("name,make", [c for c in CASES if c[0] in ("lap2d_32", "stencil27_12", "one_row")],
+                         ids=["lap2d_32", "stencil27_12", "one_row"])
+@pytest.mark.parametrize("method", [0, 1])
+def test_dia_bdia(ctx, name, make, method):
+    coo = make()
+    x, y0 = _x_y(coo)
+    if coo.width == coo.height:
+        y, _ = gpu_spmv(ctx, L.FMT_DIA, M.coo2dia(coo), x, y0, method)
+        _check(coo, y, x, y0, f"DIA m{method} {name}")
+        y, _ = gpu_spmv(ctx, L.FMT_BDIA, M.coo2bdia(coo), x, y0, method)
+        _check(coo, y, x, y0, f"BDIA m{method} {name}")
+
+
+SHAPES = [(bh, bw) for bw in (4, 8) for bh in (1, 2, 4, 8)]
+
+
+@pytest.mark.parametrize("bh,bw", SHAPES)
+@pytest.mark.parametrize("name", ["blocks3", "rand_ragged", "lap2d_32"])
+def test_block_formats(ctx, bh, bw, name):
+    coo = dict(CASES)[name]()
+    x, y0 = _x_y(coo)
+    y, _ = gpu_spmv(ctx, L.FMT_BELL, M.coo2b4ell(coo, bw, bh), x, y0)
+    _check(coo, y, x, y0, f"BELL {bh}x{bw} {name}")
+    y, _ = gpu_spmv(ctx, L.FMT_SBELL, M.coo2sbell(coo, bw, bh, 32), x, y0)
+    _check(coo, y, x, y0, f"SBELL {bh}x{bw} {name}")
+    y, _ = gpu_spmv(ctx, L.FMT_BCSR, M.coo2b4csr(coo, bw, bh), x, y0)
+    _check(coo, y, x, y0, f"BCSR {bh}x{bw} {name}")
+
+
+@pytest.mark.parametrize("b", [2, 3, 4, 5, 6])
+def test_bsr_native(ctx, b):
+    coo = M.gen_block_rows(500, b, 6, 30, 8)
+    x, y0 = _x_y(coo)
+    y, info = gpu_spmv(ctx, L.FMT_BSR, M.coo2bsr(coo, b, b), x, y0)
+    _check(coo, y, x, y0, f"BSR {b}x{b} {info['kernel']}")
+
+
+def test_repeat_accumulates(ctx):
+    # ntimes launches accumulate: y = y0 + 3 A x
+    coo = M.gen_laplacian2d(16, 16)
+    x = np.ones(coo.width, np.float32)
+    y0 = np.zeros(coo.height, np.float32)
+    y, _ = gpu_spmv(ctx, L.FMT_ELL, M.coo2ell(coo), x, y0, 0, ntimes=3)
+    ref = O.spmv_only(coo.row_id, coo.col_id, coo.data, x, coo.height)
+    assert np.array_equal(y, 3 * ref)
