@@ -154,6 +154,7 @@ _SIGS = [
     ("clspmv_part_bcsr", C.c_int, [C.c_void_p, _P(B4Csr), C.c_int, _P(C.c_void_p)]),
     ("clspmv_part_bsr", C.c_int, [C.c_void_p, _P(Bsr), C.c_int, _P(C.c_void_p)]),
     ("clspmv_part_spmv", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    ("clspmv_set_option", C.c_int, [C.c_char_p, C.c_longlong]),
     ("clspmv_part_get_info", C.c_int, [C.c_void_p, _P(PartInfo)]),
     ("clspmv_part_free", C.c_int, [C.c_void_p]),
     ("clspmv_probe_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _FP]),

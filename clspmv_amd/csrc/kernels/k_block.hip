@@ -166,6 +166,46 @@ __global__ __launch_bounds__(256) void bsr_kernel(const int* __restrict__ rp, co
     }
 }
 
+// Sliced BSR: slices of 64 block rows (one wave), per-slice width; the slot
+// of (slice s, block row t, k) is sp[s] + k*64 + t, so lane t's loads of the
+// column and of each of the BH*BW value streams are 256 contiguous bytes per
+// wave.  Padding slots repeat the row's last column with zero values.
+template <int BH, int BW>
+__global__ __launch_bounds__(256) void bsr_sliced_kernel(const int* __restrict__ sp, const int* __restrict__ col,
+                                                         const float* __restrict__ data, int stride, int brows,
+                                                         const float* __restrict__ x, float* __restrict__ y,
+                                                         int rows, int cols) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= brows) return;
+    const int s = r >> 6, t = r & 63;
+    const int b = sp[s];
+    const int width = (sp[s + 1] - b) >> 6;
+    float acc[BH];
+#pragma unroll
+    for (int h = 0; h < BH; ++h) acc[h] = 0.f;
+    for (int k = 0; k < width; ++k) {
+        const int idx = b + (k << 6) + t;
+        const int c = ld_stream(col + idx);
+        float xv[BW];
+        if (c + BW <= cols) {
+#pragma unroll
+            for (int w = 0; w < BW; ++w) xv[w] = x[c + w];
+        } else {
+#pragma unroll
+            for (int w = 0; w < BW; ++w) xv[w] = (c + w < cols) ? x[c + w] : 0.f;
+        }
+#pragma unroll
+        for (int h = 0; h < BH; ++h)
+#pragma unroll
+            for (int w = 0; w < BW; ++w) acc[h] = fmaf(ld_stream(data + (size_t)(h * BW + w) * stride + idx), xv[w], acc[h]);
+    }
+#pragma unroll
+    for (int h = 0; h < BH; ++h) {
+        const int row = r * BH + h;
+        if (row < rows) y[row] += acc[h];
+    }
+}
+
 __global__ void copy_probe_kernel(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * 256;
@@ -245,8 +285,25 @@ static void launch_bsr_l(const DevBsr& m, const float* x, float* y, hipStream_t 
     }
 }
 
+template <int BH, int BW>
+static void launch_bsr_sliced(const DevBsr& m, const float* x, float* y, hipStream_t st) {
+    hipLaunchKernelGGL((dev::bsr_sliced_kernel<BH, BW>), dim3((m.brows + 255) / 256), dim3(256), 0, st, m.row_ptr,
+                       m.col, m.data, m.stride, m.brows, x, y, m.rows, m.cols);
+}
+
 int launch_bsr(const DevBsr& m, const float* x, float* y, hipStream_t st) {
     if (m.brows == 0) return 0;
+    if (m.lanes == 0) {
+        switch (m.bh * 10 + m.bw) {
+            case 22: launch_bsr_sliced<2, 2>(m, x, y, st); break;
+            case 33: launch_bsr_sliced<3, 3>(m, x, y, st); break;
+            case 44: launch_bsr_sliced<4, 4>(m, x, y, st); break;
+            case 55: launch_bsr_sliced<5, 5>(m, x, y, st); break;
+            case 66: launch_bsr_sliced<6, 6>(m, x, y, st); break;
+            default: return (int)hipErrorInvalidValue;
+        }
+        return (int)hipGetLastError();
+    }
     switch (m.bh * 10 + m.bw) {
         case 22: launch_bsr_l<2, 2>(m, x, y, st); break;
         case 33: launch_bsr_l<3, 3>(m, x, y, st); break;

@@ -116,6 +116,71 @@ __global__ __launch_bounds__(256) void csr_adaptive_kernel(const int* __restrict
     }
 }
 
+// One column slice of the column-sliced CSR.  Same block kinds as
+// csr_adaptive_kernel; STREAM blocks take each row's entry count from one
+// byte per row (`counts`) and an exclusive wave scan gives the offsets.  All
+// x reads of a launch fall in one slice of columns, small enough to stay in
+// L2 while the entry streams pass through with nontemporal loads.
+__global__ __launch_bounds__(256) void csr_slice_kernel(const unsigned char* __restrict__ counts,
+                                                        const int* __restrict__ col, const float* __restrict__ val,
+                                                        const int4* __restrict__ blocks, int nblocks,
+                                                        const float* __restrict__ x, float* __restrict__ y,
+                                                        double* __restrict__ carry) {
+    __shared__ float lds[4][kCsrStreamMaxNnz];
+    const int wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + wid;
+    if (b >= nblocks) return;
+    const int4 blk = blocks[b];
+    const int r0 = blk.x, tag = blk.y, n0 = blk.z, n1 = blk.w;
+    if (tag < 0 || tag - r0 == 1) {
+        float acc = 0.f;
+        int k = n0 + lane;
+        for (; k + 192 < n1; k += 256) {
+            int c0 = ld_stream(col + k), c1 = ld_stream(col + k + 64), c2 = ld_stream(col + k + 128),
+                c3 = ld_stream(col + k + 192);
+            float v0 = ld_stream(val + k), v1 = ld_stream(val + k + 64), v2 = ld_stream(val + k + 128),
+                  v3 = ld_stream(val + k + 192);
+            acc = fmaf(v0, x[c0], acc);
+            acc = fmaf(v1, x[c1], acc);
+            acc = fmaf(v2, x[c2], acc);
+            acc = fmaf(v3, x[c3], acc);
+        }
+        for (; k < n1; k += 64) acc = fmaf(ld_stream(val + k), x[ld_stream(col + k)], acc);
+        acc = wave_sum(acc);
+        if (lane == 0) {
+            if (tag < 0)
+                carry[-tag - 1] = (double)acc;
+            else if (n1 > n0)
+                y[r0] += acc;
+        }
+        return;
+    }
+    const int n = n1 - n0;
+    float* sh = lds[wid];
+#pragma unroll
+    for (int u = 0; u < kCsrStreamPerLane; ++u) {
+        const int k = lane + 64 * u;
+        if (k < n) sh[k] = ld_stream(val + n0 + k) * x[ld_stream(col + n0 + k)];
+    }
+    const int r = r0 + lane;
+    const int cnt = (r < tag) ? (int)counts[r] : 0;
+    // exclusive scan of the counts across the wave
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int up = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += up;
+    }
+    const int off = incl - cnt;
+    wave_lds_sync();
+    if (cnt > 0) {
+        float acc = 0.f;
+        for (int k = off; k < off + cnt; ++k) acc += sh[k];
+        y[r] += acc;
+    }
+}
+
 __global__ __launch_bounds__(256) void csr_long_finalize_kernel(const int* __restrict__ long_rows,
                                                                 const int* __restrict__ long_first,
                                                                 const double* __restrict__ carry, int n_long,
@@ -158,6 +223,22 @@ int launch_csr_adaptive(const DevCsrAdaptive& m, const float* x, float* y, hipSt
     if (m.nblocks > 0) {
         hipLaunchKernelGGL(dev::csr_adaptive_kernel, dim3((m.nblocks + 3) / 4), dim3(256), 0, st, m.csr.row_ptr,
                            m.csr.col, m.csr.val, m.blocks, m.nblocks, x, y, m.carry);
+        int e = (int)hipGetLastError();
+        if (e) return e;
+    }
+    if (m.n_long > 0) {
+        hipLaunchKernelGGL(dev::csr_long_finalize_kernel, dim3((m.n_long + 255) / 256), dim3(256), 0, st, m.long_rows,
+                           m.long_first, m.carry, m.n_long, y);
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_csr_sliced(const DevCsrSliced& m, const float* x, float* y, hipStream_t st) {
+    for (int s = 0; s < m.nslices; ++s) {
+        const int b0 = m.slice_block_begin[s], nb = m.slice_block_begin[s + 1] - b0;
+        if (nb == 0) continue;
+        hipLaunchKernelGGL(dev::csr_slice_kernel, dim3((nb + 3) / 4), dim3(256), 0, st, m.counts + (size_t)s * m.rows,
+                           m.col, m.val, m.blocks + b0, nb, x, y, m.carry);
         int e = (int)hipGetLastError();
         if (e) return e;
     }

@@ -33,6 +33,24 @@ struct DevCsrAdaptive {
     const int* long_first;    // first carry slot of every long row (size n_long + 1)
     int n_long;
 };
+// Column-sliced CSR: the columns are cut into slices whose part of x fits in
+// one XCD's L2; slice s is processed by launch s (launches are ordered on the
+// stream and each adds into y).  Entries are stored slice-major; per slice a
+// row's entry count is one byte (rows with more than 64 entries in a slice
+// get VECTOR/LONG blocks whose descriptor carries the range).
+constexpr int kMaxColSlices = 64;
+struct DevCsrSliced {
+    const int* col;
+    const float* val;
+    const unsigned char* counts;  // nslices * rows
+    const int4* blocks;           // all slices' blocks, slice-major
+    int slice_block_begin[kMaxColSlices + 1];  // host-side block ranges per slice
+    int nslices, rows;
+    double* carry;
+    const int* long_rows;
+    const int* long_first;
+    int n_long;
+};
 struct DevCoo {
     const int* row;
     const int* col;
@@ -72,17 +90,19 @@ struct DevBcsr {  // b4csr
     int N4, brows, bh, w4n, rows, cols, lanes;
 };
 struct DevBsr {
-    const int* row_ptr;
+    const int* row_ptr;   // lanes > 0: block-CSR row pointer; lanes == 0: slice pointer
     const int* col;
     const float* data;
     int stride, brows, bh, bw, rows, cols, lanes;
 };
+constexpr int kBsrSlice = 64;  // block rows per slice of the sliced BSR layout
 
 int launch_ell(const DevEll& m, int method, const float* x, float* y, hipStream_t st);
 int launch_sell(const DevSell& m, const float* x, float* y, hipStream_t st);
 int launch_csr_scalar(const DevCsr& m, const float* x, float* y, hipStream_t st);
 int launch_csr_vector(const DevCsr& m, int lanes, const float* x, float* y, hipStream_t st);
 int launch_csr_adaptive(const DevCsrAdaptive& m, const float* x, float* y, hipStream_t st);
+int launch_csr_sliced(const DevCsrSliced& m, const float* x, float* y, hipStream_t st);
 int launch_coo(const DevCoo& m, const float* x, float* y, hipStream_t st);
 int launch_dia(const DevDia& m, int method, const float* x, float* y, hipStream_t st);
 int launch_bdia(const DevBdia& m, int method, const float* x, float* y, hipStream_t st);

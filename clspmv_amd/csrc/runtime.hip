@@ -73,6 +73,8 @@ struct clspmv_part_s {
     DevSell sell{};
     DevCsr csr{};
     DevCsrAdaptive csra{};
+    DevCsrSliced csrs{};
+    long long carry_bytes = 0;
     int csr_lanes = 0;
     DevCoo coo{};
     DevDia dia{};
@@ -141,6 +143,9 @@ int make_part(clspmv_ctx_t ctx, clspmv_part_t* out, F build) {
     *out = p;
     return CLSPMV_OK;
 }
+
+int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
+                     int cols);
 
 }  // namespace
 
@@ -245,6 +250,28 @@ int clspmv_part_ell(clspmv_ctx_t ctx, const clspmv_ell_matrix* m, int method, cl
         }
         int rc = 0;
         size_t n = (size_t)H * K;
+        if (method == 2) {
+            // Column-sliced device layout built from the ELL slots (every
+            // slot kept, padding included, in slot order = column order).
+            if ((long long)rows * K > INT32_MAX) return CLSPMV_E_TOO_LARGE;
+            std::vector<int> rp((size_t)rows + 1), ci((size_t)rows * K);
+            std::vector<float> cv((size_t)rows * K);
+            for (int r = 0; r <= rows; ++r) rp[(size_t)r] = r * K;
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+            for (int r = 0; r < rows; ++r)
+                for (int k = 0; k < K; ++k) {
+                    ci[(size_t)r * K + k] = m->ell_col_id[(size_t)k * H + r];
+                    cv[(size_t)r * K + k] = m->ell_data[(size_t)k * H + r];
+                }
+            rc = build_csr_sliced(ctx, p, rp.data(), ci.data(), cv.data(), rows, cols);
+            if (rc) return rc;
+            char name[32];
+            std::snprintf(name, sizeof(name), "ell_sliced_S%d", p->csrs.nslices);
+            fill_info(p, CLSPMV_FMT_ELL, 2, rows, cols, m->matinfo.nnz, name, p->csrs.nslices + (p->csrs.n_long ? 1 : 0));
+            p->info.stored_bytes -= p->carry_bytes;
+            p->info.alg_bytes -= p->carry_bytes;
+            return CLSPMV_OK;
+        }
         p->ell.col = (const int*)p->add(ctx, m->ell_col_id, n * 4, &rc);
         if (rc) return rc;
         p->ell.val = (const float*)p->add(ctx, m->ell_data, n * 4, &rc);
@@ -252,7 +279,7 @@ int clspmv_part_ell(clspmv_ctx_t ctx, const clspmv_ell_matrix* m, int method, cl
         p->ell.H = H;
         p->ell.ell_num = K;
         p->ell.rows = rows;
-        int meth = (method == 1 || method == 3) ? 1 : 0;  // 2,3: texture variants of 0,1
+        int meth = (method == 1 || method == 3) ? 1 : 0;  // 3: texture variant of 1
         fill_info(p, CLSPMV_FMT_ELL, meth, rows, cols, m->matinfo.nnz, meth ? "ell_row4" : "ell_row", 1);
         return CLSPMV_OK;
     });
@@ -319,6 +346,137 @@ static int upload_csr(clspmv_ctx_t ctx, clspmv_part_t p, const clspmv_csr_matrix
     return CLSPMV_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Column slices whose x range is at most ~2 MiB: small enough to stay in an
+// XCD's 4 MiB L2 next to the streaming entries.
+long long g_slice_x_bytes = 2ll << 20;  // tunable: clspmv_set_option("col_slice_bytes", v)
+
+// Builds the column-sliced device layout (DevCsrSliced) from CSR arrays whose
+// rows are sorted by column.  Blocks per slice follow the adaptive planner;
+// LONG chunks get carry slots grouped by row (a row long in several slices
+// has consecutive slots), so the finalize kernel has one writer per row.
+int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
+                     int cols) {
+    const long long nnz = rows ? rp[rows] : 0;
+    const long long sxb = std::max(256ll, g_slice_x_bytes);
+    int ns = (int)std::min<long long>(kMaxColSlices, std::max<long long>(1, (4ll * cols + sxb - 1) / sxb));
+    const int sw = std::max(1, (cols + ns - 1) / ns);  // columns per slice
+    ns = std::max(1, (cols + sw - 1) / sw);
+    std::vector<int> cnt((size_t)ns * rows, 0);
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+    for (int r = 0; r < rows; ++r)
+        for (int k = rp[r]; k < rp[r + 1]; ++k) cnt[(size_t)(ci[k] / sw) * rows + r]++;
+    // slice-major start of every (slice, row) segment
+    std::vector<long long> start((size_t)ns * rows + 1, 0);
+    {
+        long long acc = 0;
+        for (size_t i = 0; i < (size_t)ns * rows; ++i) {
+            start[i] = acc;
+            acc += cnt[i];
+        }
+        start[(size_t)ns * rows] = acc;
+    }
+    if (nnz > INT32_MAX) return CLSPMV_E_TOO_LARGE;
+    std::vector<int> scol((size_t)nnz);
+    std::vector<float> sval((size_t)nnz);
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+    for (int r = 0; r < rows; ++r) {
+        int s_prev = -1;
+        long long pos = 0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+            const int s = ci[k] / sw;
+            if (s != s_prev) {
+                pos = start[(size_t)s * rows + r];
+                s_prev = s;
+            }
+            scol[(size_t)pos] = ci[k];
+            sval[(size_t)pos] = cv[k];
+            ++pos;
+        }
+    }
+    std::vector<unsigned char> cb((size_t)ns * rows);
+    for (size_t i = 0; i < cb.size(); ++i) cb[i] = (unsigned char)std::min(255, cnt[i]);
+    // blocks
+    std::vector<std::vector<int4>> sb((size_t)ns);
+    std::vector<std::pair<int, int>> longs;  // (row, slice)
+    for (int s = 0; s < ns; ++s) {
+        const int* c = &cnt[(size_t)s * rows];
+        const long long* st = &start[(size_t)s * rows];
+        int r = 0;
+        while (r < rows) {
+            const int len = c[r];
+            if (len > kCsrStreamMaxRowLen) {
+                if (len <= kCsrLongChunk)
+                    sb[s].push_back(make_int4(r, r + 1, (int)st[r], (int)(st[r] + len)));
+                else
+                    longs.push_back({r, s});
+                ++r;
+                continue;
+            }
+            const int r0 = r;
+            int n = 0;
+            while (r < rows && r - r0 < 64) {
+                const int l = c[r];
+                if (l > kCsrStreamMaxRowLen || n + l > kCsrStreamMaxNnz) break;
+                n += l;
+                ++r;
+            }
+            sb[s].push_back(make_int4(r0, r, (int)st[r0], (int)(st[r0] + n)));
+        }
+    }
+    std::sort(longs.begin(), longs.end());
+    std::vector<int> long_rows, long_first{0};
+    int slots = 0;
+    for (size_t i = 0; i < longs.size(); ++i) {
+        const int r = longs[i].first, s = longs[i].second;
+        const long long a = start[(size_t)s * rows + r], e = a + cnt[(size_t)s * rows + r];
+        for (long long n0 = a; n0 < e; n0 += kCsrLongChunk) {
+            const long long n1 = std::min(e, n0 + kCsrLongChunk);
+            sb[s].push_back(make_int4(r, -(slots + 1), (int)n0, (int)n1));
+            ++slots;
+        }
+        if (i + 1 == longs.size() || longs[i + 1].first != r) {
+            long_rows.push_back(r);
+            long_first.push_back(slots);
+        }
+    }
+    std::vector<int4> blocks;
+    DevCsrSliced d{};
+    for (int s = 0; s < ns; ++s) {
+        d.slice_block_begin[s] = (int)blocks.size();
+        blocks.insert(blocks.end(), sb[s].begin(), sb[s].end());
+    }
+    d.slice_block_begin[ns] = (int)blocks.size();
+    int rc = 0;
+    d.col = (const int*)p->add(ctx, scol.data(), scol.size() * 4, &rc);
+    if (rc) return rc;
+    d.val = (const float*)p->add(ctx, sval.data(), sval.size() * 4, &rc);
+    if (rc) return rc;
+    d.counts = (const unsigned char*)p->add(ctx, cb.data(), cb.size(), &rc);
+    if (rc) return rc;
+    d.blocks = (const int4*)p->add(ctx, blocks.data(), blocks.size() * sizeof(int4), &rc);
+    if (rc) return rc;
+    d.long_rows = (const int*)p->add(ctx, long_rows.data(), long_rows.size() * 4, &rc);
+    if (rc) return rc;
+    d.long_first = (const int*)p->add(ctx, long_first.data(), long_first.size() * 4, &rc);
+    if (rc) return rc;
+    d.carry = (double*)p->scratch((size_t)std::max(1, slots) * 8, &rc);
+    if (rc) return rc;
+    d.nslices = ns;
+    d.rows = rows;
+    d.n_long = (int)long_rows.size();
+    p->csrs = d;
+    p->carry_bytes = (long long)std::max(1, slots) * 8;
+    return CLSPMV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int clspmv_part_csr_scalar(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int method, clspmv_part_t* out) {
     return make_part(ctx, out, [&](clspmv_part_t p) {
         int rc = upload_csr(ctx, p, m);
@@ -338,6 +496,19 @@ int clspmv_part_csr_vector(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int met
         const int rows = m->matinfo.height;
         const int* rp = m->csr_row_ptr;
         const long long nnz = rp[rows];
+        if (method == 2) {
+            rc = build_csr_sliced(ctx, p, m->csr_row_ptr, m->csr_col_id, m->csr_data, rows, m->matinfo.width);
+            if (rc) return rc;
+            // the plain CSR upload is not used by this layout
+            for (int i = 0; i < 3; ++i) p->bufs[i].release();
+            char name[32];
+            std::snprintf(name, sizeof(name), "csr_sliced_S%d", p->csrs.nslices);
+            fill_info(p, CLSPMV_FMT_CSR_VECTOR, 2, rows, m->matinfo.width, m->matinfo.nnz, name,
+                      p->csrs.nslices + (p->csrs.n_long ? 1 : 0));
+            p->info.stored_bytes -= p->carry_bytes;
+            p->info.alg_bytes -= p->carry_bytes;
+            return CLSPMV_OK;
+        }
         if (method == 0) {
             double mean = rows ? (double)nnz / rows : 0.0;
             p->csr_lanes = std::min(64, std::max(2, pow2_floor((int)(mean / 2.0 + 0.5))));
@@ -622,11 +793,57 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
             return CLSPMV_E_INVALID;
         }
         int rc = 0;
+        const unsigned bs = bw * bh;
+        if (method == 0) {
+            // Sliced device layout (see bsr_sliced_kernel): built here from
+            // the block-CSR host arrays.
+            const int S = kBsrSlice;
+            const int ns = (brows + S - 1) / S;
+            std::vector<int> sp((size_t)ns + 1, 0);
+            for (int s = 0; s < ns; ++s) {
+                int wmax = 0;
+                for (int r = s * S; r < std::min(brows, s * S + S); ++r)
+                    wmax = std::max(wmax, m->bsr_row_ptr[r + 1] - m->bsr_row_ptr[r]);
+                if ((int64_t)sp[s] + (int64_t)wmax * S > INT32_MAX) return CLSPMV_E_TOO_LARGE;
+                sp[s + 1] = sp[s] + wmax * S;
+            }
+            const size_t slots = (size_t)sp[ns];
+            const size_t sstride = std::max<size_t>(64, slots);
+            std::vector<int> scol(slots, 0);
+            std::vector<float> sdat(sstride * bs, 0.0f);
+            for (int r = 0; r < brows; ++r) {
+                const int s = r / S, t = r % S;
+                const int width = (sp[s + 1] - sp[s]) / S;
+                const int a = m->bsr_row_ptr[r], e = m->bsr_row_ptr[r + 1];
+                int last = 0;
+                for (int k = 0; k < width; ++k) {
+                    const size_t idx = (size_t)sp[s] + (size_t)k * S + t;
+                    if (a + k < e) {
+                        last = m->bsr_col_id[a + k];
+                        for (unsigned q = 0; q < bs; ++q)
+                            sdat[(size_t)q * sstride + idx] = m->bsr_data[(size_t)q * stride + a + k];
+                    }
+                    scol[idx] = last;
+                }
+            }
+            p->bsr.row_ptr = (const int*)p->add(ctx, sp.data(), sp.size() * 4, &rc);
+            if (rc) return rc;
+            p->bsr.col = (const int*)p->add(ctx, scol.data(), slots * 4, &rc);
+            if (rc) return rc;
+            p->bsr.data = (const float*)p->add(ctx, sdat.data(), sdat.size() * 4, &rc);
+            if (rc) return rc;
+            p->bsr = DevBsr{p->bsr.row_ptr, p->bsr.col, p->bsr.data, (int)sstride, brows, (int)bh, (int)bw,
+                            rows, cols, 0};
+            char name[32];
+            std::snprintf(name, sizeof(name), "bsr%u%u_sliced", bh, bw);
+            fill_info(p, CLSPMV_FMT_BSR, 0, rows, cols, m->matinfo.nnz, name, 1);
+            return CLSPMV_OK;
+        }
         p->bsr.row_ptr = (const int*)p->add(ctx, m->bsr_row_ptr, ((size_t)brows + 1) * 4, &rc);
         if (rc) return rc;
         p->bsr.col = (const int*)p->add(ctx, m->bsr_col_id, (size_t)nb * 4, &rc);
         if (rc) return rc;
-        p->bsr.data = (const float*)p->add(ctx, m->bsr_data, (size_t)stride * bw * bh * 4, &rc);
+        p->bsr.data = (const float*)p->add(ctx, m->bsr_data, (size_t)stride * bs * 4, &rc);
         if (rc) return rc;
         double mean = brows ? (double)nb / brows : 0.0;
         int lanes = (method >= 2 && method <= 16) ? pow2_floor(method) : std::min(16, std::max(2, pow2_floor((int)(mean + 0.5))));
@@ -638,14 +855,28 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
     });
 }
 
+int clspmv_set_option(const char* name, long long value) {
+    if (!name) return CLSPMV_E_INVALID;
+    if (std::strcmp(name, "col_slice_bytes") == 0) {
+        if (value < 256) return CLSPMV_E_INVALID;
+        g_slice_x_bytes = value;
+        return CLSPMV_OK;
+    }
+    set_error("unknown option %s", name);
+    return CLSPMV_E_INVALID;
+}
+
 int clspmv_part_launch(clspmv_part_t p, const float* x, float* y, hipStream_t st) {
     switch (p->info.format) {
-        case CLSPMV_FMT_ELL: return launch_ell(p->ell, p->info.method, x, y, st);
+        case CLSPMV_FMT_ELL:
+            return p->info.method == 2 ? launch_csr_sliced(p->csrs, x, y, st)
+                                       : launch_ell(p->ell, p->info.method, x, y, st);
         case CLSPMV_FMT_SELL: return launch_sell(p->sell, x, y, st);
         case CLSPMV_FMT_CSR_SCALAR: return launch_csr_scalar(p->csr, x, y, st);
         case CLSPMV_FMT_CSR_VECTOR:
-            return p->info.method == 0 ? launch_csr_vector(p->csr, p->csr_lanes, x, y, st)
-                                       : launch_csr_adaptive(p->csra, x, y, st);
+            return p->info.method == 0   ? launch_csr_vector(p->csr, p->csr_lanes, x, y, st)
+                   : p->info.method == 2 ? launch_csr_sliced(p->csrs, x, y, st)
+                                         : launch_csr_adaptive(p->csra, x, y, st);
         case CLSPMV_FMT_COO: return launch_coo(p->coo, x, y, st);
         case CLSPMV_FMT_DIA: return launch_dia(p->dia, p->info.method, x, y, st);
         case CLSPMV_FMT_BDIA: return launch_bdia(p->bdia, p->info.method, x, y, st);

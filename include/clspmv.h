@@ -309,7 +309,21 @@ int clspmv_part_sbell(clspmv_ctx_t ctx, const clspmv_sbell_matrix* m, int method
 int clspmv_part_bcsr(clspmv_ctx_t ctx, const clspmv_b4csr_matrix* m, int method, clspmv_part_t* p);
 int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, clspmv_part_t* p);
 
+/* Kernel variants (`method`) per part:
+ *   ELL      0 one row per lane, 1 four rows per lane, 2 column-sliced
+ *   CSR vec  0 L lanes per row, 1 adaptive row blocks, 2 column-sliced
+ *   DIA      0 one row per lane, 1 four rows per lane
+ *   BDIA     0 direct x reads, 1 LDS-staged x band
+ *   BSR      0 sliced (64 block rows per slice), 1 (or L = 2..16) L lanes
+ *            per block row
+ * "Column-sliced" cuts the columns into slices whose part of x (<= 2 MiB by
+ * default) stays in one XCD's L2; one launch per slice adds into y.  The
+ * reference's method ids (e.g. texture variants) map onto these. */
 int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
+/* Process-wide tuning knobs (no reference counterpart):
+ *   "col_slice_bytes"  x bytes per column slice of the column-sliced layouts
+ *                      (default 2 MiB); affects parts created afterwards. */
+int clspmv_set_option(const char* name, long long value);
 int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info);
 int clspmv_part_free(clspmv_part_t p);
 

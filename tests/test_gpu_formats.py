@@ -102,6 +102,42 @@ def test_csr_adaptive_long_rows(ctx):
         _check(coo, y, x, y0, f"CSR long rows m{method}")
 
 
+@pytest.fixture
+def small_slices():
+    # force many column slices on small matrices (256 columns per slice)
+    L.check(L.lib().clspmv_set_option(b"col_slice_bytes", 1024), "set_option")
+    yield
+    L.check(L.lib().clspmv_set_option(b"col_slice_bytes", 2 << 20), "set_option")
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=IDS)
+def test_column_sliced(ctx, small_slices, name, make):
+    coo = make()
+    x, y0 = _x_y(coo)
+    y, info = gpu_spmv(ctx, L.FMT_CSR_VECTOR, M.coo2csr(coo), x, y0, 2)
+    _check(coo, y, x, y0, f"CSR sliced {name} {info['kernel']}")
+    y, info = gpu_spmv(ctx, L.FMT_ELL, M.coo2ell(coo), x, y0, 2)
+    _check(coo, y, x, y0, f"ELL sliced {name} {info['kernel']}")
+
+
+def test_column_sliced_long_rows(ctx):
+    # rows long within several slices exercise the grouped FP64 carries
+    rng = np.random.default_rng(19)
+    h, w = 40, 200000
+    lens = np.array([150000, 0, 30000, 70, 9000] + [int(v) for v in rng.integers(0, 300, h - 5)])
+    rows = np.concatenate([np.full(int(l), r, np.int32) for r, l in enumerate(lens)])
+    cols = np.concatenate([np.sort(rng.choice(w, size=int(l), replace=False)).astype(np.int32) for l in lens])
+    vals = rng.standard_normal(rows.size).astype(np.float32)
+    coo = M.CooMatrix.from_arrays(rows, cols, vals, w, h)
+    x, y0 = _x_y(coo)
+    L.check(L.lib().clspmv_set_option(b"col_slice_bytes", 200000), "set_option")
+    try:
+        y, info = gpu_spmv(ctx, L.FMT_CSR_VECTOR, M.coo2csr(coo), x, y0, 2)
+    finally:
+        L.check(L.lib().clspmv_set_option(b"col_slice_bytes", 2 << 20), "set_option")
+    _check(coo, y, x, y0, f"CSR sliced long rows {info['kernel']}")
+
+
 @pytest.mark.parametrize("name,make", CASES, ids=IDS)
 def test_coo(ctx, name, make):
     coo = make()
@@ -153,11 +189,21 @@ def test_block_formats(ctx, bh, bw, name):
 
 
 @pytest.mark.parametrize("b", [2, 3, 4, 5, 6])
-def test_bsr_native(ctx, b):
+@pytest.mark.parametrize("method", [0, 1])
+def test_bsr_native(ctx, b, method):
     coo = M.gen_block_rows(500, b, 6, 30, 8)
     x, y0 = _x_y(coo)
-    y, info = gpu_spmv(ctx, L.FMT_BSR, M.coo2bsr(coo, b, b), x, y0)
+    y, info = gpu_spmv(ctx, L.FMT_BSR, M.coo2bsr(coo, b, b), x, y0, method)
     _check(coo, y, x, y0, f"BSR {b}x{b} {info['kernel']}")
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_bsr_ragged(ctx, method):
+    # unaligned shape (last block row/column partial) and uneven block rows
+    coo = rand_coo(301, 299, 5000, 12)
+    x, y0 = _x_y(coo)
+    y, info = gpu_spmv(ctx, L.FMT_BSR, M.coo2bsr(coo, 3, 3), x, y0, method)
+    _check(coo, y, x, y0, f"BSR ragged {info['kernel']}")
 
 
 def test_repeat_accumulates(ctx):
