@@ -85,6 +85,26 @@ class Bsr(C.Structure):
                 ("bsr_row_ptr", _IP), ("bsr_col_id", _IP), ("bsr_data", _FP)]
 
 
+class Cocktail(C.Structure):
+    _fields_ = ([(f"ifuse{n}", C.c_ubyte) for n in ("bdia", "dia", "sbell", "bell", "bcsr", "sell", "ell", "csr", "coo")]
+                + [(f"{n}_meth_num", C.c_int) for n in ("bdia", "dia", "sbell", "bell", "bcsr", "sell", "ell", "csr",
+                                                        "coo")]
+                + [("csr_group_num", C.c_int), ("coo_group_num", C.c_int), ("mat_width", C.c_int),
+                   ("mat_height", C.c_int), ("mat_nnz", C.c_int), ("bdia", Bdia), ("dia", Dia), ("sbell", Sbell),
+                   ("bell", B4Ell), ("bcsr", B4Csr), ("sell", Sell), ("ell", Ell), ("csr", Csr), ("coo", Coo)])
+
+
+class MaxflopInfo(C.Structure):
+    _fields_ = [("bdia_meth_num", C.c_int), ("bdia_max_flop", C.c_double), ("dia_meth_num", C.c_int),
+                ("dia_max_flop", C.c_double), ("sbell_meth_num", C.c_int * 8), ("sbell_max_flop", C.c_double * 8),
+                ("bell_meth_num", C.c_int * 8), ("bell_max_flop", C.c_double * 8), ("bcsr_meth_num", C.c_int * 8),
+                ("bcsr_max_flop", C.c_double * 8), ("sell_meth_num", C.c_int), ("sell_max_flop", C.c_double),
+                ("ell_meth_num", C.c_int), ("ell_max_flop", C.c_double), ("csr_meth_num", C.c_int),
+                ("csr_groupnum", C.c_int), ("csr_max_flop", C.c_double), ("coo_meth_num", C.c_int),
+                ("coo_groupnum", C.c_int), ("coo_max_flop", C.c_double), ("max_dia_base_flop", C.c_double),
+                ("max_block_base_flop", C.c_double), ("max_flat_base_flop", C.c_double)]
+
+
 class PartInfo(C.Structure):
     _fields_ = [("format", C.c_int), ("method", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
                 ("nnz", C.c_longlong), ("stored_bytes", C.c_longlong), ("alg_bytes", C.c_longlong),
@@ -157,6 +177,14 @@ _SIGS = [
     ("clspmv_set_option", C.c_int, [C.c_char_p, C.c_longlong]),
     ("clspmv_part_get_info", C.c_int, [C.c_void_p, _P(PartInfo)]),
     ("clspmv_part_free", C.c_int, [C.c_void_p]),
+    ("clspmv_init_cocktail", None, [_P(Cocktail)]),
+    ("clspmv_free_cocktail", None, [_P(Cocktail)]),
+    ("clspmv_bench_load", C.c_int, [C.c_char_p, _P(C.c_void_p)]),
+    ("clspmv_bench_free", C.c_int, [C.c_void_p]),
+    ("clspmv_bench_sizes", C.c_int, [C.c_void_p, _P(C.c_int)]),
+    ("clspmv_maxflop", C.c_int, [C.c_void_p, C.c_int, C.c_int, _P(MaxflopInfo)]),
+    ("clspmv_find_overhead", C.c_double, [C.c_void_p, C.c_int, C.c_int]),
+    ("clspmv_analyze_matrix", C.c_int, [C.c_void_p, _P(Coo), _P(Cocktail), C.c_float, C.c_int, C.c_int, C.c_int]),
     ("clspmv_probe_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _FP]),
     ("clspmv_probe_launch", C.c_int, [C.c_void_p, C.c_int, _FP]),
 ]

@@ -1,0 +1,133 @@
+/*
+ * clspmv_cocktail.h -- the Cocktail analyzer and the Cocktail GPU driver
+ * (C ABI of libclspmv.so, continued from clspmv.h).
+ *
+ * Replaces (paths relative to the reference tree, A/ = clSpMV_AMD/):
+ *   A/include/analyze.h:12-257      benchmark tables, cocktail<> struct,
+ *                                   analyze_matrix
+ *   A/src/analyze.cpp:16-210        ReadBench (reads $CLSPMVPATH/benchmark)
+ *   A/include/spmv_cocktail.h:7-175 cocktail_gpu / cocktail_kernels and the
+ *                                   init_mat_kernels / do_spmv / cpy_* API
+ */
+#ifndef CLSPMV_COCKTAIL_H_
+#define CLSPMV_COCKTAIL_H_
+
+#include "clspmv.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cocktail<int,int,float> (A/include/analyze.h:105-144), same layout: nine
+ * one-byte flags, method numbers, group counts, shape, then the parts. */
+typedef struct {
+    unsigned char ifusebdia, ifusedia, ifusesbell, ifusebell, ifusebcsr, ifusesell, ifuseell, ifusecsr, ifusecoo;
+    int bdia_meth_num, dia_meth_num, sbell_meth_num, bell_meth_num, bcsr_meth_num;
+    int sell_meth_num, ell_meth_num, csr_meth_num, coo_meth_num;
+    int csr_group_num, coo_group_num;
+    int mat_width, mat_height, mat_nnz;
+    clspmv_bdia_matrix bdia;
+    clspmv_dia_matrix dia;
+    clspmv_sbell_matrix sbell;
+    clspmv_b4ell_matrix bell;
+    clspmv_b4csr_matrix bcsr;
+    clspmv_sell_matrix sell;
+    clspmv_ell_matrix ell;
+    clspmv_csr_matrix csr;
+    clspmv_coo_matrix coo;
+} clspmv_cocktail;
+
+/* init_cocktail / free_cocktail (analyze.h:146-206). */
+void clspmv_init_cocktail(clspmv_cocktail* mat);
+void clspmv_free_cocktail(clspmv_cocktail* mat);
+
+/* Performance tables (benchmark_all, analyze.h:91-103), read from the ten
+ * .ben files of a directory in the reference's text schema
+ * (ReadBench, analyze.cpp:177-210).  dir = "<CLSPMVPATH>/benchmark". */
+typedef struct clspmv_bench_s* clspmv_bench_t;
+int clspmv_bench_load(const char* dir, clspmv_bench_t* bench);
+int clspmv_bench_free(clspmv_bench_t bench);
+/* Number of entries read per table, in the order overhead, bdia, dia,
+ * sbell, bell, bcsr, sell, ell, csr, coo. */
+int clspmv_bench_sizes(clspmv_bench_t bench, int counts[10]);
+
+/* maxflop_info (analyze.h:208-243): the interpolated best GFLOPS and method
+ * per format for a (rows, nnz/row) point; exposed for parity tests. */
+typedef struct {
+    int bdia_meth_num;
+    double bdia_max_flop;
+    int dia_meth_num;
+    double dia_max_flop;
+    int sbell_meth_num[8];
+    double sbell_max_flop[8];
+    int bell_meth_num[8];
+    double bell_max_flop[8];
+    int bcsr_meth_num[8];
+    double bcsr_max_flop[8];
+    int sell_meth_num;
+    double sell_max_flop;
+    int ell_meth_num;
+    double ell_max_flop;
+    int csr_meth_num;
+    int csr_groupnum;
+    double csr_max_flop;
+    int coo_meth_num;
+    int coo_groupnum;
+    double coo_max_flop;
+    double max_dia_base_flop;
+    double max_block_base_flop;
+    double max_flat_base_flop;
+} clspmv_maxflop_info;
+
+/* update_info (analyze.cpp:1358): all maxflop_* for rows x nnz/row, with
+ * nnz/row = (int)((float)nnz/(float)rows + 0.5). */
+int clspmv_maxflop(clspmv_bench_t bench, int rows, int nnz, clspmv_maxflop_info* info);
+/* find_overhead (analyze.cpp:413-428), microseconds. */
+double clspmv_find_overhead(clspmv_bench_t bench, int groupnum, int threadnum);
+
+/* analyze_matrix (analyze.h:257, analyze.cpp:3714-3900): splits `coo` into
+ * up to nine parts; an unsorted `coo` is sorted in place first.
+ * Verbose decision logging goes to stderr when verbose != 0. */
+int clspmv_analyze_matrix(clspmv_bench_t bench, clspmv_coo_matrix* coo, clspmv_cocktail* mat,
+                          float dia_priority_multiplier, int estimate_flat_partial, int estimate_flat_full,
+                          int verbose);
+
+/* ------------------------------------------------------------------ */
+/* Cocktail GPU driver: cocktail_gpu + cocktail_kernels
+ * (A/include/spmv_cocktail.h:7-161).  One object owns the device copies of
+ * every active part, x (padded to a multiple of 8 floats, as cpy_shared
+ * does, A/src/spmv_cocktail.cpp:230-250), y (padded to a multiple of 1024)
+ * and each part's kernel choice. */
+typedef struct clspmv_gpumat_s* clspmv_gpumat_t;
+
+/* init_mat_kernels (spmv_cocktail.cpp:1590-1611): uploads the parts, x and
+ * y; use_tex is accepted and ignored (no texture path on CDNA4). */
+int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* mat, const float* vec, const float* res,
+                            int use_tex, clspmv_gpumat_t* gpumat);
+/* free_mat_kernels (:1613-1618). */
+int clspmv_free_mat_kernels(clspmv_gpumat_t gpumat);
+/* do_spmv (:1088-1132): ntimes * (y += A x), parts in the reference's fixed
+ * order BDIA, DIA, SBELL, BELL, BCSR, SELL, ELL, CSR, COO on one stream;
+ * asynchronous (clspmv_synchronize waits). */
+int clspmv_do_spmv(clspmv_gpumat_t gpumat, int ntimes);
+/* cpy_vector_from_cpu / _from_gpu, cpy_result_from_cpu / _from_gpu /
+ * _to_cpu / _to_gpu (:1458-1588).  *_gpu take device pointers. */
+int clspmv_cpy_vector_from_cpu(clspmv_gpumat_t gpumat, const float* vec, int vec_size);
+int clspmv_cpy_vector_from_gpu(clspmv_gpumat_t gpumat, const void* d_vec, int vec_size);
+int clspmv_cpy_result_from_cpu(clspmv_gpumat_t gpumat, const float* res, int res_size);
+int clspmv_cpy_result_from_gpu(clspmv_gpumat_t gpumat, const void* d_res, int res_size);
+int clspmv_cpy_result_to_cpu(clspmv_gpumat_t gpumat, float* res, int res_size);
+int clspmv_cpy_result_to_gpu(clspmv_gpumat_t gpumat, void* d_res, int res_size);
+/* Device pointers of the object's x and y (the reference reads them as
+ * gpumat.vec / gpumat.res, e.g. A/src/eval.cpp:50-51). */
+int clspmv_gpumat_buffers(clspmv_gpumat_t gpumat, void** d_vec, void** d_res);
+/* Per-part description: count and, for part i, its format/method/kernel,
+ * nnz and stored bytes (same fields as clspmv_part_info). */
+int clspmv_gpumat_num_parts(clspmv_gpumat_t gpumat);
+int clspmv_gpumat_part_info(clspmv_gpumat_t gpumat, int i, clspmv_part_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLSPMV_COCKTAIL_H_ */

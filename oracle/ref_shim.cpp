@@ -92,3 +92,49 @@ void ref_spmv_only(void* coo, float* vec, float* res) { spmv_only((Coo*)coo, vec
 void ref_free(void* p) { std::free(p); }
 
 }  // extern "C"
+
+// Functions of A/src/analyze.cpp with external linkage that analyze.h does
+// not declare.
+void update_info(coo_matrix<int, float>& coomat, maxflop_info& info, benchmark_all& bench);
+double find_overhead(std::vector<overhead_entry>& benchover, int groupnum, int threadnum);
+void ReadBench(benchmark_all& bench);
+
+#include "clspmv_cocktail.h"
+static_assert(sizeof(cocktail<int, int, float>) == sizeof(clspmv_cocktail), "cocktail layout");
+static_assert(sizeof(maxflop_info) == sizeof(clspmv_maxflop_info), "maxflop_info layout");
+
+static benchmark_all* g_bench = nullptr;
+
+extern "C" {
+
+// analyze_matrix with the tables of $CLSPMVPATH/benchmark.
+void ref_analyze_matrix(void* coo, void* mat, float dia_mult, int partial, int full) {
+    analyze_matrix(*(Coo*)coo, *(cocktail<int, int, float>*)mat, dia_mult, partial != 0, full != 0);
+}
+
+// update_info for a (rows, nnz) point: every maxflop_* of the reference.
+void ref_maxflop(int rows, int nnz, void* out) {
+    if (!g_bench) {
+        g_bench = new benchmark_all();
+        ReadBench(*g_bench);
+    }
+    Coo m;
+    init_coo_matrix(m);
+    m.matinfo.height = rows;
+    m.matinfo.width = rows;
+    m.matinfo.nnz = nnz;
+    maxflop_info info;
+    std::memset(&info, 0, sizeof(info));
+    update_info(m, info, *g_bench);
+    std::memcpy(out, &info, sizeof(info));
+}
+
+double ref_find_overhead(int groupnum, int threadnum) {
+    if (!g_bench) {
+        g_bench = new benchmark_all();
+        ReadBench(*g_bench);
+    }
+    return find_overhead(g_bench->overhead, groupnum, threadnum);
+}
+
+}  // extern "C"
