@@ -185,6 +185,18 @@ _SIGS = [
     ("clspmv_maxflop", C.c_int, [C.c_void_p, C.c_int, C.c_int, _P(MaxflopInfo)]),
     ("clspmv_find_overhead", C.c_double, [C.c_void_p, C.c_int, C.c_int]),
     ("clspmv_analyze_matrix", C.c_int, [C.c_void_p, _P(Coo), _P(Cocktail), C.c_float, C.c_int, C.c_int, C.c_int]),
+    ("clspmv_init_mat_kernels", C.c_int, [C.c_void_p, _P(Cocktail), _FP, _FP, C.c_int, _P(C.c_void_p)]),
+    ("clspmv_free_mat_kernels", C.c_int, [C.c_void_p]),
+    ("clspmv_do_spmv", C.c_int, [C.c_void_p, C.c_int]),
+    ("clspmv_cpy_vector_from_cpu", C.c_int, [C.c_void_p, _FP, C.c_int]),
+    ("clspmv_cpy_vector_from_gpu", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    ("clspmv_cpy_result_from_cpu", C.c_int, [C.c_void_p, _FP, C.c_int]),
+    ("clspmv_cpy_result_from_gpu", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    ("clspmv_cpy_result_to_cpu", C.c_int, [C.c_void_p, _FP, C.c_int]),
+    ("clspmv_cpy_result_to_gpu", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    ("clspmv_gpumat_buffers", C.c_int, [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p)]),
+    ("clspmv_gpumat_num_parts", C.c_int, [C.c_void_p]),
+    ("clspmv_gpumat_part_info", C.c_int, [C.c_void_p, C.c_int, _P(PartInfo)]),
     ("clspmv_probe_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _FP]),
     ("clspmv_probe_launch", C.c_int, [C.c_void_p, C.c_int, _FP]),
 ]

@@ -1,0 +1,169 @@
+// Cocktail GPU driver: the device side of a Cocktail partition.
+//
+// Replaces cocktail_gpu / cocktail_kernels and their functions
+// (A/src/spmv_cocktail.cpp:17-1618).  Each active part of the partition is
+// uploaded as a device part (runtime.hip) with the kernel variant that the
+// reference's method number maps to; x and y are owned by the object, padded
+// as the reference pads them (x to a multiple of 8, y to a multiple of 1024,
+// A/src/spmv_cocktail.cpp:230-250).  do_spmv enqueues the parts in the
+// reference's fixed order on one stream, so parts that accumulate into the
+// same y are serialised as on the reference's in-order queue.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "clspmv.h"
+#include "clspmv_cocktail.h"
+#include "host/internal.hpp"
+#include "runtime.hpp"
+
+using namespace clspmv;
+
+struct clspmv_gpumat_s {
+    clspmv_ctx_t ctx = nullptr;
+    DevBuf x, y;
+    int width = 0, height = 0;
+    std::vector<clspmv_part_t> parts;  // in do_spmv order
+    ~clspmv_gpumat_s() {
+        for (auto p : parts) clspmv_part_free(p);
+        x.release();
+        y.release();
+    }
+};
+
+namespace {
+
+// Kernel variant for each reference method number.
+//  BDIA 0 gpu_bdia_nlvec, 3 gpu_bdia_v4_nlvec -> direct x reads (0);
+//       1 gpu_bdia, 2 gpu_bdia_g4, 4 gpu_bdia_v4 -> LDS-staged band (1).
+//  DIA  0/2 one row per thread -> 0; 1/3 four rows per thread -> 1.
+//  ELL  0 -> one row per lane, 1 -> four rows per lane.
+//  SELL, SBELL, BELL, BCSR: one kernel per layout.
+//  CSR  both reference methods -> adaptive row blocks (1): it needs no
+//       tuned workgroup count and balances any row-length distribution.
+//  COO  segmented reduction.
+int add_part(clspmv_gpumat_s* g, int which, const clspmv_cocktail* m) {
+    clspmv_part_t p = nullptr;
+    int rc = CLSPMV_OK;
+    switch (which) {
+        case 0: rc = clspmv_part_bdia(g->ctx, &m->bdia, m->bdia_meth_num, &p); break;
+        case 1: rc = clspmv_part_dia(g->ctx, &m->dia, m->dia_meth_num, &p); break;
+        case 2: rc = clspmv_part_sbell(g->ctx, &m->sbell, m->sbell_meth_num, &p); break;
+        case 3: rc = clspmv_part_bell(g->ctx, &m->bell, m->bell_meth_num, &p); break;
+        case 4: rc = clspmv_part_bcsr(g->ctx, &m->bcsr, m->bcsr_meth_num, &p); break;
+        case 5: rc = clspmv_part_sell(g->ctx, &m->sell, m->sell_meth_num, &p); break;
+        case 6: rc = clspmv_part_ell(g->ctx, &m->ell, m->ell_meth_num == 1 ? 1 : 0, &p); break;
+        case 7: rc = clspmv_part_csr_vector(g->ctx, &m->csr, 1, &p); break;
+        case 8: rc = clspmv_part_coo(g->ctx, &m->coo, 0, &p); break;
+        default: return CLSPMV_E_INVALID;
+    }
+    if (rc) return rc;
+    g->parts.push_back(p);
+    return CLSPMV_OK;
+}
+
+int check_size(clspmv_gpumat_t g, int n, int limit) {
+    if (!g || n < 0 || n > limit) {
+        set_error("size %d outside [0, %d]", n, limit);
+        return CLSPMV_E_INVALID;
+    }
+    return CLSPMV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* m, const float* vec, const float* res,
+                            int use_tex, clspmv_gpumat_t* out) {
+    (void)use_tex;
+    if (!ctx || !m || !out) return CLSPMV_E_INVALID;
+    auto* g = new (std::nothrow) clspmv_gpumat_s();
+    if (!g) return CLSPMV_E_NOMEM;
+    g->ctx = ctx;
+    g->width = m->mat_width;
+    g->height = m->mat_height;
+    const size_t xn = (size_t)clspmv_aligned_length(g->width, 8);
+    const size_t yn = (size_t)clspmv_aligned_length(g->height, 1024);
+    int rc = g->x.alloc(xn * 4);
+    if (!rc) rc = g->y.alloc(yn * 4);
+    if (!rc) rc = hip_status(hipMemsetAsync(g->x.ptr, 0, xn * 4, ctx->stream()), "hipMemset");
+    if (!rc) rc = hip_status(hipMemsetAsync(g->y.ptr, 0, yn * 4, ctx->stream()), "hipMemset");
+    if (!rc && vec) rc = clspmv_memcpy_h2d(ctx, g->x.ptr, vec, (size_t)g->width * 4);
+    if (!rc && res) rc = clspmv_memcpy_h2d(ctx, g->y.ptr, res, (size_t)g->height * 4);
+    const unsigned char use[9] = {m->ifusebdia, m->ifusedia, m->ifusesbell, m->ifusebell, m->ifusebcsr,
+                                  m->ifusesell, m->ifuseell, m->ifusecsr, m->ifusecoo};
+    for (int i = 0; i < 9 && !rc; ++i)
+        if (use[i]) rc = add_part(g, i, m);
+    if (!rc) rc = clspmv_synchronize(ctx);
+    if (rc) {
+        delete g;
+        *out = nullptr;
+        return rc;
+    }
+    *out = g;
+    return CLSPMV_OK;
+}
+
+int clspmv_free_mat_kernels(clspmv_gpumat_t g) {
+    if (!g) return CLSPMV_E_INVALID;
+    (void)clspmv_synchronize(g->ctx);
+    delete g;
+    return CLSPMV_OK;
+}
+
+int clspmv_do_spmv(clspmv_gpumat_t g, int ntimes) {
+    if (!g || ntimes < 0) return CLSPMV_E_INVALID;
+    hipStream_t st = g->ctx->stream();
+    const float* x = (const float*)g->x.ptr;
+    float* y = (float*)g->y.ptr;
+    for (int i = 0; i < ntimes; ++i)
+        for (auto p : g->parts) {
+            int e = clspmv_part_launch(p, x, y, st);
+            if (e) return hip_status((hipError_t)e, "do_spmv launch");
+        }
+    return CLSPMV_OK;
+}
+
+int clspmv_cpy_vector_from_cpu(clspmv_gpumat_t g, const float* vec, int n) {
+    int rc = check_size(g, n, g ? g->width : 0);
+    return rc ? rc : clspmv_memcpy_h2d(g->ctx, g->x.ptr, vec, (size_t)n * 4);
+}
+int clspmv_cpy_vector_from_gpu(clspmv_gpumat_t g, const void* d_vec, int n) {
+    int rc = check_size(g, n, g ? g->width : 0);
+    return rc ? rc : clspmv_memcpy_d2d(g->ctx, g->x.ptr, d_vec, (size_t)n * 4);
+}
+int clspmv_cpy_result_from_cpu(clspmv_gpumat_t g, const float* res, int n) {
+    int rc = check_size(g, n, g ? g->height : 0);
+    return rc ? rc : clspmv_memcpy_h2d(g->ctx, g->y.ptr, res, (size_t)n * 4);
+}
+int clspmv_cpy_result_from_gpu(clspmv_gpumat_t g, const void* d_res, int n) {
+    int rc = check_size(g, n, g ? g->height : 0);
+    return rc ? rc : clspmv_memcpy_d2d(g->ctx, g->y.ptr, d_res, (size_t)n * 4);
+}
+int clspmv_cpy_result_to_cpu(clspmv_gpumat_t g, float* res, int n) {
+    int rc = check_size(g, n, g ? g->height : 0);
+    return rc ? rc : clspmv_memcpy_d2h(g->ctx, res, g->y.ptr, (size_t)n * 4);
+}
+int clspmv_cpy_result_to_gpu(clspmv_gpumat_t g, void* d_res, int n) {
+    int rc = check_size(g, n, g ? g->height : 0);
+    return rc ? rc : clspmv_memcpy_d2d(g->ctx, d_res, g->y.ptr, (size_t)n * 4);
+}
+
+int clspmv_gpumat_buffers(clspmv_gpumat_t g, void** d_vec, void** d_res) {
+    if (!g) return CLSPMV_E_INVALID;
+    if (d_vec) *d_vec = g->x.ptr;
+    if (d_res) *d_res = g->y.ptr;
+    return CLSPMV_OK;
+}
+
+int clspmv_gpumat_num_parts(clspmv_gpumat_t g) { return g ? (int)g->parts.size() : 0; }
+
+int clspmv_gpumat_part_info(clspmv_gpumat_t g, int i, clspmv_part_info* info) {
+    if (!g || i < 0 || i >= (int)g->parts.size()) return CLSPMV_E_INVALID;
+    return clspmv_part_get_info(g->parts[(size_t)i], info);
+}
+
+}  // extern "C"

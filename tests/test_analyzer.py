@@ -1,0 +1,89 @@
+"""Bit-exact parity of the Cocktail analyzer with the reference's
+analyze_matrix (golden partitions written by
+tests/golden/make_golden_analyzer.py from the reference build), plus the
+table interpolation (maxflop_info) on a grid of points.  CPU only."""
+import ctypes as C
+import json
+import os
+
+import pytest
+
+from clspmv_amd import _lib as L
+from tests import analyzer_cases as AC
+from tests.golden.make_golden_analyzer import digest
+from tests.helpers import cocktail_arrays
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLD = json.load(open(os.path.join(HERE, "golden", "analyzer.json")))
+TABLES = os.path.join(ROOT, "clspmv_amd", "tables", "reference_amd")
+
+
+@pytest.fixture(scope="module")
+def bench():
+    b = C.c_void_p()
+    L.check(L.lib().clspmv_bench_load(TABLES.encode(), C.byref(b)), "bench_load")
+    yield b
+    L.lib().clspmv_bench_free(b)
+
+
+def test_tables_read_like_the_reference(bench):
+    sizes = (C.c_int * 10)()
+    L.lib().clspmv_bench_sizes(bench, sizes)
+    # record counts of the shipped AMD tables (one record per line)
+    assert list(sizes) == [39, 232, 232, 1016, 1016, 548, 93, 93, 588, 912]
+
+
+@pytest.mark.parametrize("rec", GOLD["maxflop"], ids=lambda r: f"{r['rows']}x{r['nnz']}")
+def test_maxflop_matches_reference(bench, rec):
+    m = L.MaxflopInfo()
+    L.check(L.lib().clspmv_maxflop(bench, rec["rows"], rec["nnz"], C.byref(m)), "maxflop")
+    for f, want in rec["info"].items():
+        got = getattr(m, f)
+        got = list(got) if hasattr(got, "__len__") else got
+        assert got == want, f
+
+
+_CASES = {}
+
+
+def _input(case, gen):
+    if case not in _CASES:
+        _CASES[case] = AC.make(gen)
+    return _CASES[case]
+
+
+@pytest.mark.parametrize("rec", GOLD["partitions"], ids=lambda r: f"{r['case']}-{r['knobs']}")
+def test_partition_matches_reference(bench, rec):
+    coo = _input(rec["case"], rec["gen"])
+    assert AC.input_digest(coo) == rec["input_sha256"], "generated input changed"
+    mat = L.Cocktail()
+    L.lib().clspmv_init_cocktail(C.byref(mat))
+    c = coo.copy()
+    m, p, f = rec["knobs"]
+    L.check(L.lib().clspmv_analyze_matrix(bench, C.byref(c.s), C.byref(mat), m, p, f, 0), "analyze")
+    got = digest(cocktail_arrays(mat))
+    L.lib().clspmv_free_cocktail(C.byref(mat))
+    want = rec["result"]
+    assert sorted(got) == sorted(want)
+    bad = [k for k in want if got[k] != want[k]]
+    assert not bad, bad
+
+
+def test_partition_conserves_nonzeros(bench):
+    coo = AC.make(["cocktail", 65536, 6])
+    mat = L.Cocktail()
+    L.lib().clspmv_init_cocktail(C.byref(mat))
+    c = coo.copy()
+    L.check(L.lib().clspmv_analyze_matrix(bench, C.byref(c.s), C.byref(mat), 1.0, 0, 0, 0), "analyze")
+    total = 0
+    for n in ("bdia", "dia", "sbell", "bell", "bcsr", "sell", "ell", "csr", "coo"):
+        if getattr(mat, f"ifuse{n}"):
+            total += getattr(mat, n).matinfo.nnz
+    assert total == coo.nnz
+    L.lib().clspmv_free_cocktail(C.byref(mat))
+
+
+def test_missing_tables_is_an_io_error():
+    b = C.c_void_p()
+    assert L.lib().clspmv_bench_load(b"/nonexistent", C.byref(b)) == L.E_IO
