@@ -1,0 +1,306 @@
+// LDS x-tiled SpMV (y += A*x) for matrices whose columns are scattered over a
+// wide x (uniform random columns, power-law rows).
+//
+// A random-column SpMV that reads x from global memory costs one L2 request
+// per nonzero; on MI355X that request rate, not HBM, bounds it.  This layout
+// moves the reuse of x into LDS instead:
+//   * a row block (<= 16384 rows, balanced by nonzeros) is owned by one
+//     1024-thread workgroup, which keeps the block's y partial sums in LDS;
+//   * the columns are cut into 16384-wide slices; for every slice in which
+//     the block has entries (a "tile"; a slice with more than 32 chunks is
+//     cut into several tiles) the workgroup stages x[slice] into LDS with
+//     coalesced float4 loads.  Three tiles ahead of use, their x slices and
+//     entry chunks are already in flight in three register sets (the tile
+//     loop is unrolled by three so that each set stays in place), which
+//     covers the HBM/MALL latency that a one-tile prefetch leaves exposed;
+//   * a tile's entries are sorted by (row, col), packed as
+//     (row_in_block << 16 | col_in_slice) plus the value, and padded to
+//     128-entry chunks.  A wavefront reduces one chunk (2 entries per lane)
+//     with a segmented shuffle scan; a segment that starts inside the chunk
+//     is added to its row by that wavefront alone, the chunk's first segment
+//     (which may continue an earlier chunk) goes to a per-chunk carry that is
+//     added after the next barrier, runs of equal rows in chunk order.
+//   * when x is wider than one XCD's L2 can hold, the slices are split into
+//     G = 2, 4 or 8 column groups of <= 2 MiB of x; workgroup b works on
+//     row block b / G and group b % G, so (workgroups being dispatched
+//     round-robin over the 8 XCDs) every XCD streams only its own group's x
+//     and keeps it in L2.  Each workgroup then writes its partial sums to
+//     partial[group][row] and a second kernel adds the G partials to y in
+//     group order.
+// No atomics: the summation order is fixed by the layout, so results are
+// deterministic.  Every workgroup streams all of x it touches through LDS,
+// so the x traffic is (row blocks) x (slice bytes) from L2/MALL, in
+// full cache lines, against one L2 request per nonzero for a gather.
+#include "kcommon.hpp"
+#include "launch.hpp"
+
+namespace clspmv {
+namespace dev {
+
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+
+// x[c .. c+3] is read as one vector from min(c, cols - 4) (cols >= 4 is
+// required by the host), so the load is unconditional and never leaves x.
+// Lanes past the end then hold x shifted by d = c - (cols - 4); x_store4
+// shifts the valid elements back (elements at or past cols are never read).
+__device__ __forceinline__ f32x4u load_x4(const float* __restrict__ x, int c, int cols) {
+    return *reinterpret_cast<const f32x4u*>(x + min(c, cols - 4));
+}
+__device__ __forceinline__ float4 x_store4(f32x4u v, int c, int cols) {
+    const int d = c - (cols - 4);
+    float4 o;
+    o.x = d <= 0 ? v.x : (d == 1 ? v.y : (d == 2 ? v.z : v.w));
+    o.y = d <= 0 ? v.y : (d == 1 ? v.z : v.w);
+    o.z = d <= 0 ? v.z : v.w;
+    o.w = v.w;
+    return o;
+}
+
+// Adds the run of equal-row carries that starts at slot i (slots in chunk
+// order; row -1 marks an unused slot).
+__device__ __forceinline__ void xt_apply_carry(const int* cr, const float* cv, float* y_l, int i) {
+    const int r = cr[i];
+    if (r < 0) return;
+    if (i > 0 && cr[i - 1] == r) return;
+    float s = y_l[r];
+    for (int k = i; k < kXtWaves && cr[k] == r; ++k) s += cv[k];
+    y_l[r] = s;
+}
+
+// Emits the sum of a finished segment: the chunk's first row goes to the
+// chunk's carry slot, other rows are added to y_l (padding is dropped).
+__device__ __forceinline__ void xt_emit(int row, float v, int first_row, float* y_l, int* cr, float* cv, int wave) {
+    if (row == first_row) {
+        cr[wave] = row;
+        cv[wave] = v;
+    } else if (row != (int)kXtPadRow) {
+        y_l[row] += v;
+    }
+}
+
+// One wavefront reduces one 128-entry chunk (2 entries per lane).  Writes
+// y_l for segments that start inside the chunk and the chunk's carry slot
+// for its first row.  Each segment is emitted once, by the lane holding its
+// last entry.
+__device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, float* y_l, int* cr, float* cv,
+                                         int wave, int lane) {
+    const int ra = (int)(e.x >> 16), rb = (int)(e.y >> 16);
+    const float pa = v.x * x_l[e.x & 0xFFFFu];
+    const float pb = v.y * x_l[e.y & 0xFFFFu];
+    const int head_row = ra;
+    const bool brk = ra != rb;
+    const float head_sum = pa;
+    const int tail_row = rb;
+    const float tail_sum = brk ? pb : pa + pb;
+    const int first_row = __shfl(head_row, 0, 64);
+
+    const int prev_tail_row = __shfl_up(tail_row, 1, 64);
+    const float prev_tail_sum = __shfl_up(tail_sum, 1, 64);
+    const bool cont = lane > 0 && prev_tail_row == head_row;  // head continues the previous lane's tail
+    const bool through = cont && !brk;                        // the whole lane is inside that segment
+    const unsigned long long tmask = __ballot(through);
+    const unsigned long long cmask = __ballot(cont);
+    // A segment spans three or more lanes exactly when some lane L lies
+    // inside it (through) and lane L + 1 continues it.
+    if ((tmask & (cmask >> 1)) == 0) {
+        // Every segment spans at most two lanes: one neighbour exchange
+        // completes it (the common case for short rows).
+        if (brk) xt_emit(head_row, head_sum + (cont ? prev_tail_sum : 0.f), first_row, y_l, cr, cv, wave);
+        const bool next_cont = lane < 63 && ((cmask >> (lane + 1)) & 1ull);
+        if (!next_cont)
+            xt_emit(tail_row, tail_sum + (through ? prev_tail_sum : 0.f), first_row, y_l, cr, cv, wave);
+        return;
+    }
+
+    // General case: segmented inclusive scan of the lane tails.
+    int flag = (lane == 0) || brk || (prev_tail_row != tail_row);
+    float cval = tail_sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float up_v = __shfl_up(cval, d, 64);
+        const int up_f = __shfl_up(flag, d, 64);
+        if (lane >= d && !flag) {
+            cval += up_v;
+            flag = up_f;
+        }
+    }
+    const float prev_c = __shfl_up(cval, 1, 64);
+    // a segment that ended exactly before this lane
+    const bool boundary_end = lane > 0 && prev_tail_row != head_row;
+    if (boundary_end) xt_emit(prev_tail_row, prev_c, first_row, y_l, cr, cv, wave);
+    if (brk) xt_emit(head_row, head_sum + ((lane > 0 && !boundary_end) ? prev_c : 0.f), first_row, y_l, cr, cv, wave);
+    if (lane == 63) xt_emit(tail_row, cval, first_row, y_l, cr, cv, wave);
+}
+
+// Registers holding one tile ahead of use: its x slice (unless the tile
+// continues the previous tile's slice) and this wavefront's chunks.
+struct XtSet {
+    f32x4u x[kXtCols / (4 * kXtThreads)];
+    u32x2 e[kXtRounds];
+    f32x2 v[kXtRounds];
+};
+
+struct XtArgs {
+    const int* blk_row;
+    const int* blk_tile;
+    const int* tile_slice;  // (slice << 1) | continues-previous-slice
+    const int* tile_chunk;
+    const u32x2* idx2;
+    const f32x2* val2;
+    int cols, last_chunk;
+    const float* x;
+    float* y;
+};
+
+// Loads are issued unconditionally (indices clamped to valid ones) and each
+// register set is refilled in place right after its last use, so that the
+// wait before each use covers only the loads issued before it.
+__device__ __forceinline__ void xt_fetch_x(XtSet& S, const XtArgs& a, int t, int tid) {
+    const int base = (a.tile_slice[t] >> 1) * kXtCols;
+#pragma unroll
+    for (int q = 0; q < kXtCols / (4 * kXtThreads); ++q) S.x[q] = load_x4(a.x, base + (q * kXtThreads + tid) * 4, a.cols);
+}
+__device__ __forceinline__ void xt_fetch_e(XtSet& S, const XtArgs& a, int t, int wave, int lane) {
+    const int c0 = a.tile_chunk[t];
+#pragma unroll
+    for (int k = 0; k < kXtRounds; ++k) {
+        const int c = min(c0 + wave + k * kXtWaves, a.last_chunk);
+        S.e[k] = __builtin_nontemporal_load(a.idx2 + (size_t)c * 64 + lane);
+        S.v[k] = __builtin_nontemporal_load(a.val2 + (size_t)c * 64 + lane);
+    }
+}
+
+struct XtState {
+    int par;
+    bool pending;
+};
+
+// One tile: stage its x slice from set S, refill S with tile t + kXtDepth,
+// reduce its chunks.  Steps past the block's last tile (t >= t1) only keep
+// the barrier sequence and the carry bookkeeping, so that the unrolled loop
+// issues every load on every path.
+__device__ __forceinline__ void xt_step(XtSet& S, const XtArgs& a, int t, int t1, float* y_l, float4* x_l4,
+                                        int (*c_row)[kXtWaves], float (*c_val)[kXtWaves], XtState& st, int tid,
+                                        int wave, int lane) {
+    __syncthreads();  // previous tile done with x_l; its carry slots are written
+    if (st.pending && tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
+    st.pending = false;
+    const bool live = t < t1;
+    const int tt = live ? t : t1 - 1;
+    const int ts = a.tile_slice[tt];
+    if (live && !(ts & 1)) {
+        const int base = (ts >> 1) * kXtCols;
+        if (base + kXtCols <= a.cols) {
+#pragma unroll
+            for (int q = 0; q < kXtCols / (4 * kXtThreads); ++q) {
+                const f32x4u v = S.x[q];
+                x_l4[q * kXtThreads + tid] = make_float4(v.x, v.y, v.z, v.w);
+            }
+        } else {  // the slice holding the end of x
+#pragma unroll
+            for (int q = 0; q < kXtCols / (4 * kXtThreads); ++q)
+                x_l4[q * kXtThreads + tid] = x_store4(S.x[q], base + (q * kXtThreads + tid) * 4, a.cols);
+        }
+    }
+    const int c0 = a.tile_chunk[tt], c1 = live ? a.tile_chunk[tt + 1] : c0;
+    const int tn = min(t + kXtDepth, t1 - 1);
+    xt_fetch_x(S, a, tn, tid);
+    __syncthreads();  // x_l holds the slice; carries applied
+    const float* x_l = reinterpret_cast<const float*>(x_l4);
+#pragma unroll
+    for (int k = 0; k < kXtRounds; ++k) {
+        const int cb = c0 + k * kXtWaves;
+        if (cb < c1) {
+            if (k > 0) {
+                __syncthreads();
+                if (tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
+            }
+            if (cb + wave < c1)
+                xt_chunk(S.e[k], S.v[k], x_l, y_l, c_row[st.par], c_val[st.par], wave, lane);
+            else if (lane == 0)
+                c_row[st.par][wave] = -1;
+            st.par ^= 1;
+            st.pending = true;
+        }
+    }
+    xt_fetch_e(S, a, tn, wave, lane);
+}
+
+__global__ __launch_bounds__(kXtThreads) void xtile_kernel(
+    const int* __restrict__ blk_row, const int* __restrict__ blk_tile, const int* __restrict__ tile_slice,
+    const int* __restrict__ tile_chunk, const u32x2* __restrict__ idx2, const f32x2* __restrict__ val2, int cols,
+    int last_chunk, const float* __restrict__ x, float* __restrict__ y, float* __restrict__ partial, int groups,
+    int rows) {
+    // the metadata is read-only and unaliased, so its wave-uniform reads
+    // become scalar loads that do not wait on the vector-load queue
+    const XtArgs a{blk_row, blk_tile, tile_slice, tile_chunk, idx2, val2, cols, last_chunk, x, y};
+    __shared__ float y_l[kXtRows];
+    __shared__ float4 x_l4[kXtCols / 4];
+    __shared__ int c_row[2][kXtWaves];
+    __shared__ float c_val[2][kXtWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.x;
+    const int rb = b / groups;  // row block; column group b % groups
+    const int r0 = a.blk_row[rb], nr = a.blk_row[rb + 1] - r0;
+    const int t0 = a.blk_tile[b], t1 = a.blk_tile[b + 1];
+    float* out = groups > 1 ? partial + (size_t)(b % groups) * rows + r0 : nullptr;
+    if (t0 == t1) {  // no entries in this row block and group (uniform per workgroup)
+        if (out)
+            for (int r = tid; r < nr; r += kXtThreads) out[r] = 0.f;
+        return;
+    }
+    for (int r = tid; r < nr; r += kXtThreads) y_l[r] = 0.f;
+
+    static_assert(kXtDepth == 3, "the tile loop below is unrolled for three register sets");
+    XtSet A, B, C;
+    xt_fetch_x(A, a, t0, tid);
+    xt_fetch_e(A, a, t0, wave, lane);
+    xt_fetch_x(B, a, min(t0 + 1, t1 - 1), tid);
+    xt_fetch_e(B, a, min(t0 + 1, t1 - 1), wave, lane);
+    xt_fetch_x(C, a, min(t0 + 2, t1 - 1), tid);
+    xt_fetch_e(C, a, min(t0 + 2, t1 - 1), wave, lane);
+    XtState st{0, false};
+    for (int t = t0; t < t1; t += 3) {
+        xt_step(A, a, t, t1, y_l, x_l4, c_row, c_val, st, tid, wave, lane);
+        xt_step(B, a, t + 1, t1, y_l, x_l4, c_row, c_val, st, tid, wave, lane);
+        xt_step(C, a, t + 2, t1, y_l, x_l4, c_row, c_val, st, tid, wave, lane);
+    }
+    __syncthreads();
+    if (st.pending && tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
+    __syncthreads();
+    if (out) {
+        for (int r = tid; r < nr; r += kXtThreads) out[r] = y_l[r];
+    } else {
+        for (int r = tid; r < nr; r += kXtThreads) a.y[r0 + r] += y_l[r];
+    }
+}
+
+// y[r] += partial[0][r] + partial[1][r] + ... in group order.
+__global__ __launch_bounds__(256) void xtile_reduce_kernel(const float* __restrict__ partial, int groups, int rows,
+                                                           float* __restrict__ y) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
+    float s = partial[r];
+    for (int g = 1; g < groups; ++g) s += partial[(size_t)g * rows + r];
+    y[r] += s;
+}
+
+}  // namespace dev
+
+int launch_xtile(const DevXtile& m, const float* x, float* y, hipStream_t st) {
+    if (m.nblocks == 0) return 0;
+    hipLaunchKernelGGL(dev::xtile_kernel, dim3(m.nblocks), dim3(kXtThreads), 0, st, m.blk_row, m.blk_tile,
+                       m.tile_slice, m.tile_chunk, reinterpret_cast<const dev::u32x2*>(m.idx),
+                       reinterpret_cast<const dev::f32x2*>(m.val), m.cols, m.nchunks - 1, x, y, m.partial, m.groups,
+                       m.rows);
+    int e = (int)hipGetLastError();
+    if (e || m.groups == 1) return e;
+    hipLaunchKernelGGL(dev::xtile_reduce_kernel, dim3((m.rows + 255) / 256), dim3(256), 0, st, m.partial, m.groups,
+                       m.rows, y);
+    return (int)hipGetLastError();
+}
+
+}  // namespace clspmv

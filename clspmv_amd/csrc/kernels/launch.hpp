@@ -97,6 +97,32 @@ struct DevBsr {
 };
 constexpr int kBsrSlice = 64;  // block rows per slice of the sliced BSR layout
 
+// LDS x-tiled layout (k_xtile.hip): row blocks of <= kXtRows rows, column
+// slices of kXtCols columns, entries packed (row_in_block << 16 |
+// col_in_slice) in chunks of kXtChunk; padding entries carry row kXtPadRow.
+constexpr int kXtThreads = 1024;
+constexpr int kXtWaves = kXtThreads / 64;
+constexpr int kXtRows = 16384;
+constexpr int kXtCols = 16384;
+constexpr int kXtChunk = 128;
+constexpr int kXtRounds = 2;                       // chunk rounds per tile held in registers
+constexpr int kXtTileChunks = kXtRounds * kXtWaves;  // a slice with more chunks is split
+constexpr int kXtDepth = 3;                        // tiles prefetched ahead
+constexpr unsigned kXtPadRow = 0xFFFFu;
+constexpr int kXtGroupSlices = 32;  // column slices per group: 2 MiB of x, resident in one XCD's L2
+struct DevXtile {
+    const int* blk_row;     // nblocks + 1 row boundaries
+    const int* blk_tile;    // nblocks + 1 tile ranges
+    const int* tile_slice;  // (column slice << 1) | 1 if the tile continues the previous tile's slice
+    const int* tile_chunk;  // ntiles + 1 chunk ranges, at most kXtTileChunks per tile
+    const unsigned* idx;
+    const float* val;
+    float* partial;         // groups * rows partial sums when groups > 1
+    int nblocks;            // workgroups = row blocks * groups
+    int groups;             // column groups (1, 2, 4 or 8); workgroup b works on group b % groups
+    int ntiles, nchunks, cols, rows;
+};
+
 int launch_ell(const DevEll& m, int method, const float* x, float* y, hipStream_t st);
 int launch_sell(const DevSell& m, const float* x, float* y, hipStream_t st);
 int launch_csr_scalar(const DevCsr& m, const float* x, float* y, hipStream_t st);
@@ -110,6 +136,7 @@ int launch_bell(const DevBell& m, const float* x, float* y, hipStream_t st);
 int launch_sbell(const DevSbell& m, const float* x, float* y, hipStream_t st);
 int launch_bcsr(const DevBcsr& m, const float* x, float* y, hipStream_t st);
 int launch_bsr(const DevBsr& m, const float* x, float* y, hipStream_t st);
+int launch_xtile(const DevXtile& m, const float* x, float* y, hipStream_t st);
 int launch_copy_probe(const float4* src, float4* dst, size_t n4, hipStream_t st);
 int launch_empty(hipStream_t st);
 

@@ -83,6 +83,7 @@ struct clspmv_part_s {
     DevSbell sbell{};
     DevBcsr bcsr{};
     DevBsr bsr{};
+    DevXtile xt{};
 
     void* add(clspmv_ctx_t c, const void* host, size_t bytes, int* rc) {
         bufs.emplace_back();
@@ -146,6 +147,46 @@ int make_part(clspmv_ctx_t ctx, clspmv_part_t* out, F build) {
 
 int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
                      int cols);
+int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows, int cols);
+
+// Row-list view (CSR arrays) of a stored format, for the layouts that are
+// rebuilt from rows (column-sliced, x-tiled).  Every stored slot is kept,
+// padding included, in slot order (= column order for the reference's
+// conversions).
+struct RowLists {
+    std::vector<int> rp, ci;
+    std::vector<float> cv;
+};
+
+int ell_rows(const clspmv_ell_matrix* m, RowLists* o) {
+    const int rows = m->matinfo.height, K = m->ell_num, H = m->ell_height_aligned;
+    if ((long long)rows * K > INT32_MAX) return CLSPMV_E_TOO_LARGE;
+    o->rp.resize((size_t)rows + 1);
+    o->ci.resize((size_t)rows * K);
+    o->cv.resize((size_t)rows * K);
+    for (int r = 0; r <= rows; ++r) o->rp[(size_t)r] = r * K;
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+    for (int r = 0; r < rows; ++r)
+        for (int k = 0; k < K; ++k) {
+            o->ci[(size_t)r * K + k] = m->ell_col_id[(size_t)k * H + r];
+            o->cv[(size_t)r * K + k] = m->ell_data[(size_t)k * H + r];
+        }
+    return CLSPMV_OK;
+}
+
+// Method number of the LDS x-tiled layout, accepted by the ELL, SELL, CSR
+// and COO parts.
+constexpr int kMethodXtile = 4;
+
+int finish_xtile(clspmv_part_t p, int fmt, int rows, int cols, long long nnz) {
+    char name[48];
+    std::snprintf(name, sizeof(name), "xtile_B%d_G%d", p->xt.nblocks / p->xt.groups, p->xt.groups);
+    fill_info(p, fmt, kMethodXtile, rows, cols, nnz, name, p->xt.groups > 1 ? 2 : 1);
+    // the partial-sum buffer is scratch, not format storage
+    p->info.stored_bytes -= p->carry_bytes;
+    p->info.alg_bytes -= p->carry_bytes;
+    return CLSPMV_OK;
+}
 
 }  // namespace
 
@@ -250,20 +291,17 @@ int clspmv_part_ell(clspmv_ctx_t ctx, const clspmv_ell_matrix* m, int method, cl
         }
         int rc = 0;
         size_t n = (size_t)H * K;
+        if (method == kMethodXtile) {
+            RowLists rl;
+            if ((rc = ell_rows(m, &rl))) return rc;
+            if ((rc = build_xtile(ctx, p, rl.rp.data(), rl.ci.data(), rl.cv.data(), rows, cols))) return rc;
+            return finish_xtile(p, CLSPMV_FMT_ELL, rows, cols, m->matinfo.nnz);
+        }
         if (method == 2) {
-            // Column-sliced device layout built from the ELL slots (every
-            // slot kept, padding included, in slot order = column order).
-            if ((long long)rows * K > INT32_MAX) return CLSPMV_E_TOO_LARGE;
-            std::vector<int> rp((size_t)rows + 1), ci((size_t)rows * K);
-            std::vector<float> cv((size_t)rows * K);
-            for (int r = 0; r <= rows; ++r) rp[(size_t)r] = r * K;
-#pragma omp parallel for schedule(static) num_threads(omp_threads())
-            for (int r = 0; r < rows; ++r)
-                for (int k = 0; k < K; ++k) {
-                    ci[(size_t)r * K + k] = m->ell_col_id[(size_t)k * H + r];
-                    cv[(size_t)r * K + k] = m->ell_data[(size_t)k * H + r];
-                }
-            rc = build_csr_sliced(ctx, p, rp.data(), ci.data(), cv.data(), rows, cols);
+            // Column-sliced device layout built from the ELL slots.
+            RowLists rl;
+            if ((rc = ell_rows(m, &rl))) return rc;
+            rc = build_csr_sliced(ctx, p, rl.rp.data(), rl.ci.data(), rl.cv.data(), rows, cols);
             if (rc) return rc;
             char name[32];
             std::snprintf(name, sizeof(name), "ell_sliced_S%d", p->csrs.nslices);
@@ -305,6 +343,28 @@ int clspmv_part_sell(clspmv_ctx_t ctx, const clspmv_sell_matrix* m, int method, 
                 }
         }
         int rc = 0;
+        if (method == kMethodXtile) {
+            // rows of the slices, padding slots included
+            RowLists rl;
+            rl.rp.assign((size_t)rows + 1, 0);
+            for (int s = 0; s < ns; ++s) {
+                const int w = (m->sell_slice_ptr[s + 1] - m->sell_slice_ptr[s]) / S;
+                for (int r = s * S; r < std::min(rows, s * S + S); ++r) rl.rp[(size_t)r + 1] = rl.rp[(size_t)r] + w;
+            }
+            rl.ci.resize((size_t)rl.rp[(size_t)rows]);
+            rl.cv.resize((size_t)rl.rp[(size_t)rows]);
+            for (int s = 0; s < ns; ++s) {
+                const int a = m->sell_slice_ptr[s], w = (m->sell_slice_ptr[s + 1] - a) / S;
+                for (int r = s * S; r < std::min(rows, s * S + S); ++r)
+                    for (int j = 0; j < w; ++j) {
+                        const size_t src = (size_t)a + (size_t)j * S + (size_t)(r - s * S);
+                        rl.ci[(size_t)rl.rp[(size_t)r] + j] = m->sell_col_id[src];
+                        rl.cv[(size_t)rl.rp[(size_t)r] + j] = m->sell_data[src];
+                    }
+            }
+            if ((rc = build_xtile(ctx, p, rl.rp.data(), rl.ci.data(), rl.cv.data(), rows, cols))) return rc;
+            return finish_xtile(p, CLSPMV_FMT_SELL, rows, cols, m->matinfo.nnz);
+        }
         size_t total = (size_t)m->sell_slice_ptr[ns];
         p->sell.slice_ptr = (const int*)p->add(ctx, m->sell_slice_ptr, ((size_t)ns + 1) * 4, &rc);
         if (rc) return rc;
@@ -473,6 +533,124 @@ int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int
     return CLSPMV_OK;
 }
 
+int g_xt_blocks = 256;     // target row blocks: one per CU (tunable: "xtile_blocks")
+// Column groups at most (tunable: "xtile_max_groups", 1, 2, 4 or 8).  One
+// group is the default: on a 4M x 4M uniform matrix grouping by XCD was
+// slower at every density (the partial-sum pass and the shorter workgroups
+// cost more than the L2 hits gain).
+int g_xt_max_groups = 1;
+
+// Builds the LDS x-tiled layout (DevXtile) from row lists.  Row blocks are
+// balanced by entry count (at most kXtRows rows each); within a block the
+// entries are bucketed by column slice in row order, so each tile is sorted
+// by (row, col) when the rows are.
+int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
+                int cols) {
+    const long long nnz = rows ? rp[rows] : 0;
+    if (cols < 4) {
+        set_error("the x-tiled layout needs at least 4 columns");
+        return CLSPMV_E_UNSUPPORTED;
+    }
+    const int ns = std::max(1, (cols + kXtCols - 1) / kXtCols);
+    std::vector<int> brow{0};
+    const long long target = std::max<long long>(1, (nnz + g_xt_blocks - 1) / g_xt_blocks);
+    for (int r = 0; r < rows;) {
+        const int rb = r;
+        long long n = 0;
+        do {
+            n += rp[r + 1] - rp[r];
+            ++r;
+        } while (r < rows && r - rb < kXtRows && n < target);
+        brow.push_back(r);
+    }
+    const int nb = (int)brow.size() - 1;
+    // per block: (slice, entries) of its nonempty slices
+    std::vector<std::vector<std::pair<int, int>>> bt((size_t)nb);
+#pragma omp parallel num_threads(omp_threads())
+    {
+        std::vector<int> cnt((size_t)ns);
+#pragma omp for schedule(dynamic, 4)
+        for (int b = 0; b < nb; ++b) {
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (long long k = rp[brow[b]]; k < rp[brow[b + 1]]; ++k) cnt[(size_t)(ci[k] / kXtCols)]++;
+            for (int s = 0; s < ns; ++s)
+                if (cnt[(size_t)s]) bt[(size_t)b].push_back({s, cnt[(size_t)s]});
+        }
+    }
+    // Column groups: G (a power of two <= 8) groups of at most
+    // kXtGroupSlices slices, when x is wider than that.  Workgroup
+    // b = block * G + group.
+    int G = 1;
+    while (G < g_xt_max_groups && (long long)G * kXtGroupSlices < ns) G *= 2;
+    const int spg = (ns + G - 1) / G;
+    // A slice's chunks are cut into tiles of at most kXtTileChunks; the
+    // later pieces are marked as continuing the slice (x stays in LDS).
+    std::vector<int> blk_tile{0}, tile_slice, tile_chunk{0};
+    for (int b = 0; b < nb; ++b) {
+        for (int g = 0; g < G; ++g) {
+            for (auto& t : bt[(size_t)b]) {
+                if (t.first / spg != g) continue;
+                const long long nc = (t.second + kXtChunk - 1) / kXtChunk;
+                if (((long long)tile_chunk.back() + nc) * kXtChunk > INT32_MAX) return CLSPMV_E_TOO_LARGE;
+                for (long long k = 0; k < nc; k += kXtTileChunks) {
+                    tile_slice.push_back((t.first << 1) | (k ? 1 : 0));
+                    tile_chunk.push_back(tile_chunk.back() + (int)std::min<long long>(kXtTileChunks, nc - k));
+                }
+            }
+            blk_tile.push_back((int)tile_slice.size());
+        }
+    }
+    if ((long long)nb * G > INT32_MAX / 2) return CLSPMV_E_TOO_LARGE;
+    const size_t total = (size_t)tile_chunk.back() * kXtChunk;
+    std::vector<unsigned> idx(std::max<size_t>(total, 1), kXtPadRow << 16);
+    std::vector<float> val(std::max<size_t>(total, 1), 0.f);
+#pragma omp parallel num_threads(omp_threads())
+    {
+        std::vector<long long> pos((size_t)ns, -1);
+#pragma omp for schedule(dynamic, 4)
+        for (int b = 0; b < nb; ++b) {
+            // entries of a slice run contiguously over its pieces
+            for (int t = blk_tile[(size_t)b * G]; t < blk_tile[(size_t)(b + 1) * G]; ++t)
+                if (!(tile_slice[(size_t)t] & 1))
+                    pos[(size_t)(tile_slice[(size_t)t] >> 1)] = (long long)tile_chunk[(size_t)t] * kXtChunk;
+            for (int r = brow[b]; r < brow[b + 1]; ++r)
+                for (long long k = rp[r]; k < rp[r + 1]; ++k) {
+                    const int s = ci[k] / kXtCols;
+                    const long long q = pos[(size_t)s]++;
+                    idx[(size_t)q] = ((unsigned)(r - brow[b]) << 16) | (unsigned)(ci[k] - s * kXtCols);
+                    val[(size_t)q] = cv[k];
+                }
+        }
+    }
+    int rc = 0;
+    DevXtile d{};
+    d.blk_row = (const int*)p->add(ctx, brow.data(), brow.size() * 4, &rc);
+    if (rc) return rc;
+    d.blk_tile = (const int*)p->add(ctx, blk_tile.data(), blk_tile.size() * 4, &rc);
+    if (rc) return rc;
+    d.tile_slice = (const int*)p->add(ctx, tile_slice.data(), std::max<size_t>(1, tile_slice.size()) * 4, &rc);
+    if (rc) return rc;
+    d.tile_chunk = (const int*)p->add(ctx, tile_chunk.data(), tile_chunk.size() * 4, &rc);
+    if (rc) return rc;
+    d.idx = (const unsigned*)p->add(ctx, idx.data(), idx.size() * 4, &rc);
+    if (rc) return rc;
+    d.val = (const float*)p->add(ctx, val.data(), val.size() * 4, &rc);
+    if (rc) return rc;
+    if (G > 1) {
+        d.partial = (float*)p->scratch((size_t)G * rows * 4, &rc);
+        if (rc) return rc;
+        p->carry_bytes = (long long)G * rows * 4;
+    }
+    d.groups = G;
+    d.nblocks = nb * G;
+    d.ntiles = (int)tile_slice.size();
+    d.nchunks = tile_chunk.back();
+    d.cols = cols;
+    d.rows = rows;
+    p->xt = d;
+    return CLSPMV_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -496,6 +674,12 @@ int clspmv_part_csr_vector(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int met
         const int rows = m->matinfo.height;
         const int* rp = m->csr_row_ptr;
         const long long nnz = rp[rows];
+        if (method == kMethodXtile) {
+            rc = build_xtile(ctx, p, m->csr_row_ptr, m->csr_col_id, m->csr_data, rows, m->matinfo.width);
+            if (rc) return rc;
+            for (int i = 0; i < 3; ++i) p->bufs[i].release();  // plain CSR upload unused
+            return finish_xtile(p, CLSPMV_FMT_CSR_VECTOR, rows, m->matinfo.width, m->matinfo.nnz);
+        }
         if (method == 2) {
             rc = build_csr_sliced(ctx, p, m->csr_row_ptr, m->csr_col_id, m->csr_data, rows, m->matinfo.width);
             if (rc) return rc;
@@ -581,6 +765,14 @@ int clspmv_part_coo(clspmv_ctx_t ctx, const clspmv_coo_matrix* m, int method, cl
                 set_error("COO entries must be sorted by row");
                 return CLSPMV_E_INVALID;
             }
+        }
+        if (method == kMethodXtile) {
+            std::vector<int> rp((size_t)rows + 1, 0);
+            for (int k = 0; k < nnz; ++k) rp[(size_t)m->coo_row_id[k] + 1]++;
+            for (int r = 0; r < rows; ++r) rp[(size_t)r + 1] += rp[(size_t)r];
+            int rc = build_xtile(ctx, p, rp.data(), m->coo_col_id, m->coo_data, rows, cols);
+            if (rc) return rc;
+            return finish_xtile(p, CLSPMV_FMT_COO, rows, cols, nnz);
         }
         const int chunks = (nnz + kCooChunk - 1) / kCooChunk;
         const size_t padded = (size_t)chunks * kCooChunk;
@@ -857,6 +1049,16 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
 
 int clspmv_set_option(const char* name, long long value) {
     if (!name) return CLSPMV_E_INVALID;
+    if (std::strcmp(name, "xtile_max_groups") == 0) {
+        if (value != 1 && value != 2 && value != 4 && value != 8) return CLSPMV_E_INVALID;
+        g_xt_max_groups = (int)value;
+        return CLSPMV_OK;
+    }
+    if (std::strcmp(name, "xtile_blocks") == 0) {
+        if (value < 1 || value > (1 << 20)) return CLSPMV_E_INVALID;
+        g_xt_blocks = (int)value;
+        return CLSPMV_OK;
+    }
     if (std::strcmp(name, "col_slice_bytes") == 0) {
         if (value < 256) return CLSPMV_E_INVALID;
         g_slice_x_bytes = value;
@@ -867,6 +1069,10 @@ int clspmv_set_option(const char* name, long long value) {
 }
 
 int clspmv_part_launch(clspmv_part_t p, const float* x, float* y, hipStream_t st) {
+    if (p->info.method == kMethodXtile &&
+        (p->info.format == CLSPMV_FMT_ELL || p->info.format == CLSPMV_FMT_SELL ||
+         p->info.format == CLSPMV_FMT_CSR_VECTOR || p->info.format == CLSPMV_FMT_COO))
+        return launch_xtile(p->xt, x, y, st);
     switch (p->info.format) {
         case CLSPMV_FMT_ELL:
             return p->info.method == 2 ? launch_csr_sliced(p->csrs, x, y, st)
