@@ -5,16 +5,18 @@ Default workload (c1, the single-format FP32 kernels of the headline metric):
 one step = one y += A x on each of four seeded matrices, each in the format it
 is meant to exercise, all resident in HBM:
     c1a 27-point stencil 128^3            DIA
-    c1b 4M rows x 32 uniform columns      ELL
-    c1c 2M rows Zipf(1.8) mean 16         CSR (adaptive row blocks)
-    c1d 1M block rows of 3x3 blocks x 9   BSR 3x3
+    c1b 4M rows x 32 uniform columns      ELL, LDS x-tiled layout
+    c1c 2M rows Zipf(1.8) mean 16         CSR, LDS x-tiled layout
+    c1d 1M block rows of 3x3 blocks x 9   BSR 3x3, sliced layout
 value = sum(2*nnz) / step time (GFLOP/s).  Each kernel is also timed alone
 with HIP events on the context stream; the dominant one is reported against
 the HBM roofline with its algorithmic bytes (format arrays + x + y read and
-write).  The reference's CPU SpMV (spmv_only -> coo_spmv, one core) is timed
-on the host on a bounded sample of the same matrices.
+write).  The Cocktail composite (c2, 4M rows) is then analyzed, uploaded and
+timed beside the best single format (extra field "cocktail_c2").  The
+reference's CPU SpMV (spmv_only -> coo_spmv, one core) is timed on the host
+on a bounded sample of the same c1 matrices.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-c2] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -54,7 +56,7 @@ def cpu_baseline(workloads, budget_s=12.0):
         y = np.zeros(coo.height, np.float32)
         reps = 0
         t_w = 0.0
-        while reps == 0 or (t_w < budget_s / len(workloads) and reps < 5):
+        while reps == 0 or (t_w < budget_s / len(workloads) and reps < 100):
             t0 = time.perf_counter()
             if R is not None:
                 R.ref_spmv_only(C.addressof(coo.s), x.ctypes.data_as(C.POINTER(C.c_float)),
@@ -76,6 +78,69 @@ def cpu_baseline(workloads, budget_s=12.0):
     }
 
 
+def cocktail_c2(ctx, args):
+    """The Cocktail composite (4M rows: band + 4x4 blocks + Zipf rows):
+    analyze with the reference's performance tables, upload every part, time
+    do_spmv; beside it the best single format (CSR, x-tiled or
+    column-sliced) on the same matrix."""
+    import numpy as np
+
+    from clspmv_amd import _lib as L
+    from clspmv_amd import cocktail as CK
+    from clspmv_amd import matrix as M
+    from clspmv_amd import workloads as W
+
+    coo = W.make_coo("c2", args.scale)
+    t0 = time.perf_counter()
+    tables = CK.BenchTables(CK.REFERENCE_TABLES)
+    mat = CK.analyze_matrix(coo.copy(), tables)
+    t_an = time.perf_counter() - t0
+    x = M.gen_vector(coo.width, 7)
+    gpu = CK.GpuCocktail(ctx, mat, x, np.zeros(coo.height, np.float32))
+    parts = gpu.part_infos()
+    k = max(10, args.steps)
+    gpu.do_spmv(2)
+    ctx.sync()
+    ctx.time_begin()
+    gpu.do_spmv(k)
+    ms = ctx.time_end() / k
+    stored = sum(p["stored_bytes"] for p in parts)
+    alg = stored + 4 * coo.width + 8 * coo.height
+    out = {
+        "rows": coo.height, "nnz": int(coo.nnz), "analysis_s": round(t_an, 2),
+        "partition": mat.describe()["parts"],
+        "kernels": [p["kernel"] for p in parts],
+        "ms": round(ms, 5), "gflops": round(2.0 * coo.nnz / (ms * 1e-3) / 1e9, 1),
+        "alg_bytes": int(alg), "gbps": round(alg / (ms * 1e-3) / 1e9, 1),
+        "pct_hbm_peak": round(100.0 * alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1),
+    }
+    gpu.free()
+    mat.free()
+    best = None
+    csr = M.coo2csr(coo)
+    dx = M.DeviceArray.from_numpy(ctx, x)
+    dy = M.DeviceArray.from_numpy(ctx, np.zeros(coo.height, np.float32))
+    for meth in (4, 2):
+        part = M.Part(ctx, L.FMT_CSR_VECTOR, csr, meth)
+        part.spmv(dx, dy, 2)
+        ctx.sync()
+        ctx.time_begin()
+        part.spmv(dx, dy, k)
+        m = ctx.time_end() / k
+        if best is None or m < best["ms"]:
+            best = {"format": "CSR", "kernel": part.info["kernel"], "ms": round(m, 5),
+                    "gflops": round(2.0 * coo.nnz / (m * 1e-3) / 1e9, 1)}
+        part.free()
+    csr.free()
+    dx.free()
+    dy.free()
+    out["best_single_format"] = best
+    out["cocktail_vs_best_single"] = round(best["ms"] / ms, 3)
+    log(f"c2 Cocktail: {out['partition']} {ms * 1e3:.1f} us {out['gflops']} GFLOP/s "
+        f"(best single {best['kernel']} {best['ms'] * 1e3:.1f} us), analysis {t_an:.1f} s")
+    return out
+
+
 def load_traffic(kernel):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -94,6 +159,7 @@ def main():
     ap.add_argument("--workload", default="c1", choices=["c1"])
     ap.add_argument("--scale", type=int, default=1, help="divide matrix sizes (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c2", action="store_true", help="skip the Cocktail composite leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported "
@@ -167,6 +233,8 @@ def main():
         "kernel": d["kernel"], "matrix": dom, "alg_bytes_per_launch": d["alg_bytes"],
     }
 
+    c2 = None if args.no_c2 else cocktail_c2(ctx, args)
+
     cpu = None
     if not args.no_cpu:
         log("timing the reference CPU SpMV (1 core)...")
@@ -193,8 +261,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generators, resident in HBM)",
         "config": {
-            "workload": "c1: one FP32 y+=Ax on each of c1a 27-pt stencil 128^3 (DIA), c1b 4M x 32 uniform (ELL column-sliced), "
-                        "c1c 2M Zipf(1.8) mean 16 max 50k (CSR column-sliced), c1d 1M block rows 3x3 x 9 (BSR sliced)"
+            "workload": "c1: one FP32 y+=Ax on each of c1a 27-pt stencil 128^3, c1b 4M x 32 uniform, "
+                        "c1c 2M Zipf(1.8) mean 16 max 50k, c1d 1M block rows 3x3 x 9; formats: "
+                        + ", ".join(f"{w['name']} {w['desc']}" for w in work)
                         + (f" [sizes / {args.scale}]" if args.scale > 1 else ""),
             "matrices": names,
             "nnz_total": int(sum(w["coo"].nnz for w in work)),
@@ -205,6 +274,7 @@ def main():
         "per_matrix": per,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cocktail_c2": c2,
     }
     print(json.dumps(out), flush=True)
 

@@ -10,6 +10,8 @@
 // same y are serialised as on the reference's in-order queue.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -35,6 +37,27 @@ struct clspmv_gpumat_s {
 
 namespace {
 
+constexpr int kXtile = 4;  // method number of the x-tiled layout
+
+// Layout choice for the row-list parts (ELL, SELL, CSR, COO) when x is wider
+// than one XCD's L2 (4 MiB): the x-tiled layout (method 4) when its staging
+// cost is below the gather cost, else the column-sliced layout (ELL, CSR) or
+// the part's own kernel.  Costs fitted on MI355X (uniform random columns,
+// 4M x 4M): gather ~ 26 us per 2 MiB column slice + 4.65 ps per entry;
+// x-tiled ~ staged bytes / 9.5 TB/s + 1.35 ps per entry, where every row
+// block (one per CU, at most 16384 rows) stages each 64 KiB slice it touches.
+bool wide_x(int cols) { return 4.0 * cols > 4.0 * 1024 * 1024; }
+bool xtile_pays(long long rows, long long cols, long long nnz) {
+    if (nnz <= 0 || cols < 4) return false;
+    const double ns = std::ceil(cols / 16384.0);
+    const double nb = std::max(std::min(256.0, (double)nnz), std::ceil(rows / 16384.0));
+    const double per_block = nnz / nb;
+    const double staged = nb * ns * (1.0 - std::exp(-per_block / ns)) * 65536.0;
+    const double t_xt = staged / 9.5e12 + 1.35e-12 * nnz;
+    const double t_gather = 26e-6 * std::ceil(4.0 * cols / (2.0 * 1024 * 1024)) + 4.65e-12 * nnz;
+    return t_xt < t_gather;
+}
+
 // Kernel variant for each reference method number.
 //  BDIA 0 gpu_bdia_nlvec, 3 gpu_bdia_v4_nlvec -> direct x reads (0);
 //       1 gpu_bdia, 2 gpu_bdia_g4, 4 gpu_bdia_v4 -> LDS-staged band (1).
@@ -44,6 +67,7 @@ namespace {
 //  CSR  both reference methods -> adaptive row blocks (1): it needs no
 //       tuned workgroup count and balances any row-length distribution.
 //  COO  segmented reduction.
+//  Wide x: see xtile_pays above.
 int add_part(clspmv_gpumat_s* g, int which, const clspmv_cocktail* m) {
     clspmv_part_t p = nullptr;
     int rc = CLSPMV_OK;
@@ -53,10 +77,31 @@ int add_part(clspmv_gpumat_s* g, int which, const clspmv_cocktail* m) {
         case 2: rc = clspmv_part_sbell(g->ctx, &m->sbell, m->sbell_meth_num, &p); break;
         case 3: rc = clspmv_part_bell(g->ctx, &m->bell, m->bell_meth_num, &p); break;
         case 4: rc = clspmv_part_bcsr(g->ctx, &m->bcsr, m->bcsr_meth_num, &p); break;
-        case 5: rc = clspmv_part_sell(g->ctx, &m->sell, m->sell_meth_num, &p); break;
-        case 6: rc = clspmv_part_ell(g->ctx, &m->ell, m->ell_meth_num == 1 ? 1 : 0, &p); break;
-        case 7: rc = clspmv_part_csr_vector(g->ctx, &m->csr, 1, &p); break;
-        case 8: rc = clspmv_part_coo(g->ctx, &m->coo, 0, &p); break;
+        case 5: {
+            const auto& a = m->sell.matinfo;
+            const bool xt = wide_x(a.width) && xtile_pays(a.height, a.width, a.nnz);
+            rc = clspmv_part_sell(g->ctx, &m->sell, xt ? kXtile : m->sell_meth_num, &p);
+            break;
+        }
+        case 6: {
+            const auto& a = m->ell.matinfo;
+            const int meth = !wide_x(a.width) ? (m->ell_meth_num == 1 ? 1 : 0)
+                             : xtile_pays(a.height, a.width, a.nnz) ? kXtile : 2;
+            rc = clspmv_part_ell(g->ctx, &m->ell, meth, &p);
+            break;
+        }
+        case 7: {
+            const auto& a = m->csr.matinfo;
+            const int meth = !wide_x(a.width) ? 1 : xtile_pays(a.height, a.width, a.nnz) ? kXtile : 2;
+            rc = clspmv_part_csr_vector(g->ctx, &m->csr, meth, &p);
+            break;
+        }
+        case 8: {
+            const auto& a = m->coo.matinfo;
+            const bool xt = wide_x(a.width) && xtile_pays(a.height, a.width, a.nnz);
+            rc = clspmv_part_coo(g->ctx, &m->coo, xt ? kXtile : 0, &p);
+            break;
+        }
         default: return CLSPMV_E_INVALID;
     }
     if (rc) return rc;

@@ -58,9 +58,9 @@ def best_format(name: str, coo):
     if name in ("c0", "c1a"):
         return L.FMT_DIA, 0, M.coo2dia(coo), "DIA"
     if name == "c1b":
-        return L.FMT_ELL, 2, M.coo2ell(coo), "ELL (column-sliced)"
+        return L.FMT_ELL, 4, M.coo2ell(coo), "ELL (LDS x-tiled)"
     if name == "c1c":
-        return L.FMT_CSR_VECTOR, 2, M.coo2csr(coo), "CSR (column-sliced adaptive)"
+        return L.FMT_CSR_VECTOR, 4, M.coo2csr(coo), "CSR (LDS x-tiled)"
     if name == "c1d":
         return L.FMT_BSR, 0, M.coo2bsr(coo, 3, 3), "BSR 3x3 (sliced)"
     raise KeyError(name)
