@@ -310,19 +310,32 @@ int clspmv_part_bcsr(clspmv_ctx_t ctx, const clspmv_b4csr_matrix* m, int method,
 int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, clspmv_part_t* p);
 
 /* Kernel variants (`method`) per part:
- *   ELL      0 one row per lane, 1 four rows per lane, 2 column-sliced
- *   CSR vec  0 L lanes per row, 1 adaptive row blocks, 2 column-sliced
+ *   ELL      0 one row per lane, 1 four rows per lane, 2 column-sliced,
+ *            4 LDS x-tiled
+ *   SELL     0 one row per lane, 4 LDS x-tiled
+ *   CSR vec  0 L lanes per row, 1 adaptive row blocks, 2 column-sliced,
+ *            4 LDS x-tiled
+ *   COO      0 segmented reduction, 4 LDS x-tiled
  *   DIA      0 one row per lane, 1 four rows per lane
  *   BDIA     0 direct x reads, 1 LDS-staged x band
  *   BSR      0 sliced (64 block rows per slice), 1 (or L = 2..16) L lanes
  *            per block row
  * "Column-sliced" cuts the columns into slices whose part of x (<= 2 MiB by
- * default) stays in one XCD's L2; one launch per slice adds into y.  The
+ * default) stays in one XCD's L2; one launch per slice adds into y.
+ * "LDS x-tiled" gives each 1024-thread workgroup a block of <= 16384 rows
+ * whose partial sums stay in LDS and streams x through LDS in 16384-column
+ * slices (for wide x with scattered columns; needs width >= 4).  The
  * reference's method ids (e.g. texture variants) map onto these. */
 int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
-/* Process-wide tuning knobs (no reference counterpart):
- *   "col_slice_bytes"  x bytes per column slice of the column-sliced layouts
- *                      (default 2 MiB); affects parts created afterwards. */
+/* Process-wide tuning knobs (no reference counterpart); each affects parts
+ * created afterwards:
+ *   "col_slice_bytes"   x bytes per column slice of the column-sliced
+ *                       layouts (default 2 MiB);
+ *   "xtile_blocks"      target row blocks of the x-tiled layout (default
+ *                       256, one per CU);
+ *   "xtile_max_groups"  1, 2, 4 or 8 column groups of the x-tiled layout
+ *                       (default 1; > 1 assigns each XCD one group of x and
+ *                       adds a partial-sum pass). */
 int clspmv_set_option(const char* name, long long value);
 int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info);
 int clspmv_part_free(clspmv_part_t p);

@@ -22,7 +22,7 @@
  *                         used to check each layout's indexing.
  *   oracle_two_vec_compare  the reference's report (A/src/util.cpp:93-131).
  *
- * How each function is pinned (tests/test_oracle.py): closed forms (5-point
+ * How each function is pinned (tests/test_io_oracle_abi.py): closed forms (5-point
  * Laplacian with x = 1 gives 0 / 1 / 2 exactly; a stencil with unit values
  * and x = 1 gives the neighbour count), brute-force dense products on small
  * matrices, agreement with the reference's own compiled spmv_only
