@@ -80,9 +80,9 @@ def cpu_baseline(workloads, budget_s=12.0):
 
 def cocktail_c2(ctx, args):
     """The Cocktail composite (4M rows: band + 4x4 blocks + Zipf rows):
-    analyze with the reference's performance tables, upload every part, time
-    do_spmv; beside it the best single format (CSR, x-tiled or
-    column-sliced) on the same matrix."""
+    analyze with the reference's performance tables and with the MI355X
+    tables, upload every part, time do_spmv; beside it the best single
+    format (CSR, x-tiled or column-sliced) on the same matrix."""
     import numpy as np
 
     from clspmv_amd import _lib as L
@@ -91,31 +91,32 @@ def cocktail_c2(ctx, args):
     from clspmv_amd import workloads as W
 
     coo = W.make_coo("c2", args.scale)
-    t0 = time.perf_counter()
-    tables = CK.BenchTables(CK.REFERENCE_TABLES)
-    mat = CK.analyze_matrix(coo.copy(), tables)
-    t_an = time.perf_counter() - t0
     x = M.gen_vector(coo.width, 7)
-    gpu = CK.GpuCocktail(ctx, mat, x, np.zeros(coo.height, np.float32))
-    parts = gpu.part_infos()
     k = max(10, args.steps)
-    gpu.do_spmv(2)
-    ctx.sync()
-    ctx.time_begin()
-    gpu.do_spmv(k)
-    ms = ctx.time_end() / k
-    stored = sum(p["stored_bytes"] for p in parts)
-    alg = stored + 4 * coo.width + 8 * coo.height
-    out = {
-        "rows": coo.height, "nnz": int(coo.nnz), "analysis_s": round(t_an, 2),
-        "partition": mat.describe()["parts"],
-        "kernels": [p["kernel"] for p in parts],
-        "ms": round(ms, 5), "gflops": round(2.0 * coo.nnz / (ms * 1e-3) / 1e9, 1),
-        "alg_bytes": int(alg), "gbps": round(alg / (ms * 1e-3) / 1e9, 1),
-        "pct_hbm_peak": round(100.0 * alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1),
-    }
-    gpu.free()
-    mat.free()
+    out = {"rows": coo.height, "nnz": int(coo.nnz)}
+    for label, path in (("reference_tables", CK.REFERENCE_TABLES), ("mi355x_tables", CK.MI355X_TABLES)):
+        t0 = time.perf_counter()
+        mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(path))
+        t_an = time.perf_counter() - t0
+        gpu = CK.GpuCocktail(ctx, mat, x, np.zeros(coo.height, np.float32))
+        parts = gpu.part_infos()
+        gpu.do_spmv(2)
+        ctx.sync()
+        ctx.time_begin()
+        gpu.do_spmv(k)
+        ms = ctx.time_end() / k
+        alg = sum(p["stored_bytes"] for p in parts) + 4 * coo.width + 8 * coo.height
+        out[label] = {
+            "analysis_s": round(t_an, 2), "partition": mat.describe()["parts"],
+            "kernels": [p["kernel"] for p in parts],
+            "ms": round(ms, 5), "gflops": round(2.0 * coo.nnz / (ms * 1e-3) / 1e9, 1),
+            "alg_bytes": int(alg), "gbps": round(alg / (ms * 1e-3) / 1e9, 1),
+            "pct_hbm_peak": round(100.0 * alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1),
+        }
+        log(f"c2 Cocktail ({label}): {out[label]['partition']} {out[label]['kernels']} {ms * 1e3:.1f} us "
+            f"{out[label]['gflops']} GFLOP/s, analysis {t_an:.1f} s")
+        gpu.free()
+        mat.free()
     best = None
     csr = M.coo2csr(coo)
     dx = M.DeviceArray.from_numpy(ctx, x)
@@ -135,9 +136,9 @@ def cocktail_c2(ctx, args):
     dx.free()
     dy.free()
     out["best_single_format"] = best
-    out["cocktail_vs_best_single"] = round(best["ms"] / ms, 3)
-    log(f"c2 Cocktail: {out['partition']} {ms * 1e3:.1f} us {out['gflops']} GFLOP/s "
-        f"(best single {best['kernel']} {best['ms'] * 1e3:.1f} us), analysis {t_an:.1f} s")
+    for label in ("reference_tables", "mi355x_tables"):
+        out[label]["vs_best_single"] = round(best["ms"] / out[label]["ms"], 3)
+    log(f"c2 best single format: {best['kernel']} {best['ms'] * 1e3:.1f} us")
     return out
 
 

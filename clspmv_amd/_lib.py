@@ -199,6 +199,7 @@ _SIGS = [
     ("clspmv_gpumat_part_info", C.c_int, [C.c_void_p, C.c_int, _P(PartInfo)]),
     ("clspmv_probe_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _FP]),
     ("clspmv_probe_launch", C.c_int, [C.c_void_p, C.c_int, _FP]),
+    ("clspmv_probe_launch_grid", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _FP]),
 ]
 
 _lib = None

@@ -41,8 +41,7 @@ constexpr int kXtile = 4;  // method number of the x-tiled layout
 
 // Layout choice for the row-list parts (ELL, SELL, CSR, COO) when x is wider
 // than one XCD's L2 (4 MiB): the x-tiled layout (method 4) when its staging
-// cost is below the gather cost, else the column-sliced layout (ELL, CSR) or
-// the part's own kernel.  Costs fitted on MI355X (uniform random columns,
+// cost is below the gather cost, else the column-sliced layout (method 2).  Costs fitted on MI355X (uniform random columns,
 // 4M x 4M): gather ~ 26 us per 2 MiB column slice + 4.65 ps per entry;
 // x-tiled ~ staged bytes / 9.5 TB/s + 1.35 ps per entry, where every row
 // block (one per CU, at most 16384 rows) stages each 64 KiB slice it touches.
@@ -79,8 +78,8 @@ int add_part(clspmv_gpumat_s* g, int which, const clspmv_cocktail* m) {
         case 4: rc = clspmv_part_bcsr(g->ctx, &m->bcsr, m->bcsr_meth_num, &p); break;
         case 5: {
             const auto& a = m->sell.matinfo;
-            const bool xt = wide_x(a.width) && xtile_pays(a.height, a.width, a.nnz);
-            rc = clspmv_part_sell(g->ctx, &m->sell, xt ? kXtile : m->sell_meth_num, &p);
+            const int meth = !wide_x(a.width) ? m->sell_meth_num : xtile_pays(a.height, a.width, a.nnz) ? kXtile : 2;
+            rc = clspmv_part_sell(g->ctx, &m->sell, meth, &p);
             break;
         }
         case 6: {
@@ -98,8 +97,8 @@ int add_part(clspmv_gpumat_s* g, int which, const clspmv_cocktail* m) {
         }
         case 8: {
             const auto& a = m->coo.matinfo;
-            const bool xt = wide_x(a.width) && xtile_pays(a.height, a.width, a.nnz);
-            rc = clspmv_part_coo(g->ctx, &m->coo, xt ? kXtile : 0, &p);
+            const int meth = !wide_x(a.width) ? 0 : xtile_pays(a.height, a.width, a.nnz) ? kXtile : 2;
+            rc = clspmv_part_coo(g->ctx, &m->coo, meth, &p);
             break;
         }
         default: return CLSPMV_E_INVALID;

@@ -273,6 +273,13 @@ double overhead_ms(const clspmv_bench_s& B, int groups, int threads) { return ov
 // nnz per row as the reference computes it everywhere: in float, +0.5, cast.
 int nnz_per_row(int nnz, int rows) { return (int)(((float)nnz) / ((float)rows) + 0.5); }
 
+// A part whose slot count does not fit the int32 slice pointers of its
+// format (the reference would overflow and write out of bounds there).
+struct TooLarge {
+    const char* what;
+    explicit TooLarge(const char* w) : what(w) {}
+};
+
 // ------------------------------------------------------------ COO helper
 
 struct Coo {
@@ -551,9 +558,12 @@ static SellEllPlan plan_sell_ell(const clspmv_bench_s& B, const clspmv_maxflop_i
         p.slice_partial_nnz += current;
     }
     const double sover = overhead_ms(B, ns, kSellGroup);
-    int full = 0, part = 0;
-    for (int v : p.slice_full) full += S * v;
-    for (int v : p.slice_partial) part += S * v;
+    // Slot totals in 64 bits: the reference sums them in int, which
+    // overflows (undefined) for matrices with very long rows; wherever it
+    // does not overflow the value is the same.
+    long long full = 0, part = 0;
+    for (int v : p.slice_full) full += (long long)S * v;
+    for (int v : p.slice_partial) part += (long long)S * v;
     p.sell_full_time = ((double)full) / 1000000.0 / in.sell_max_flop + sover;
     p.sell_partial_time = ((double)part) / 1000000.0 / in.sell_max_flop + sover +
                           ((double)(nnz - p.slice_partial_nnz)) / 1000000.0 / rflop + rover;
@@ -1017,6 +1027,9 @@ void Analyzer::extract_sbell_part(clspmv_coo_matrix& coo, Coo& rem, const std::v
     s.sbell_slice_height = (unsigned)S;
     s.sbell_slice_num = ns;
     s.sbell_row_num = brows;
+    long long slots = 0;
+    for (int v : sn) slots += (long long)v * S;
+    if (slots > INT32_MAX) throw TooLarge("SBELL part");
     s.sbell_slice_ptr = xmalloc<int>((size_t)ns + 1);
     s.sbell_slice_ptr[0] = 0;
     for (int i = 0; i < ns; ++i) s.sbell_slice_ptr[i + 1] = s.sbell_slice_ptr[i] + sn[(size_t)i] * S;
@@ -1162,8 +1175,8 @@ void Analyzer::extract_full_sbell_bell(clspmv_coo_matrix& coo, Coo& rem, const s
     const int S = kWarp;
     std::vector<int> full = slice_max(bc, S);
     const double sover = overhead_ms(B, brownum / kSellGroup, kSellGroup);
-    int sfull = 0;
-    for (int v : full) sfull += S * v;
+    long long sfull = 0;  // 64-bit, see plan_sell_ell
+    for (int v : full) sfull += (long long)S * v;
     const double sbell_full_time = ((double)(sfull * BW[wh] * BH[wh])) / 1000000.0 / info.sbell_max_flop[wh] + sover;
     log("sbell full time %f bell time %f\n", sbell_full_time, bell_time);
     if (bell_time < sbell_full_time) {
@@ -1237,9 +1250,9 @@ void Analyzer::extract_sbell_bell(clspmv_coo_matrix& coo, Coo& rem, const std::v
         partial_nnz += cur;
     }
     const double sover = overhead_ms(B, brownum / kSellGroup, kSellGroup);
-    int sfull = 0, spart = 0;
-    for (int v : full) sfull += S * v;
-    for (int v : partial) spart += S * v;
+    long long sfull = 0, spart = 0;  // 64-bit, see plan_sell_ell
+    for (int v : full) sfull += (long long)S * v;
+    for (int v : partial) spart += (long long)S * v;
     const double full_time = ((double)(sfull * BW[wh] * BH[wh])) / 1000000.0 / info.sbell_max_flop[wh] + sover +
                              ((double)(nnz - full_nnz)) / 1000000.0 / rflop + rover;
     const double part_time = ((double)(spart * BW[wh] * BH[wh])) / 1000000.0 / info.sbell_max_flop[wh] + sover +
@@ -1419,6 +1432,9 @@ void Analyzer::extract_sell(clspmv_coo_matrix& coo, Coo& rem, const std::vector<
     s.matinfo.nnz = slice_nnz;
     s.sell_slice_height = (unsigned)S;
     s.sell_slice_num = ns;
+    long long slots = 0;
+    for (int v : se) slots += (long long)v * S;
+    if (slots > INT32_MAX) throw TooLarge("SELL part");
     s.sell_slice_ptr = xmalloc<int>((size_t)ns + 1);
     s.sell_slice_ptr[0] = 0;
     for (int i = 0; i < ns; ++i) s.sell_slice_ptr[i + 1] = s.sell_slice_ptr[i] + se[(size_t)i] * S;
@@ -1730,7 +1746,13 @@ int clspmv_analyze_matrix(clspmv_bench_t b, clspmv_coo_matrix* coo, clspmv_cockt
     if ((rc = row_ptr_of(coo, rp))) return rc;
     if ((rc = check_cols(coo))) return rc;
     Analyzer a(*b, *mat, verbose != 0);
-    a.run(*coo, dia_mult, est_partial != 0, est_full != 0);
+    try {
+        a.run(*coo, dia_mult, est_partial != 0, est_full != 0);
+    } catch (const TooLarge& e) {
+        clspmv_free_cocktail(mat);
+        set_error("analyze_matrix: %s exceeds 2^31 stored entries", e.what);
+        return CLSPMV_E_TOO_LARGE;
+    }
     return CLSPMV_OK;
 }
 

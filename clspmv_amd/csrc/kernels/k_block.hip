@@ -320,8 +320,8 @@ int launch_copy_probe(const float4* src, float4* dst, size_t n4, hipStream_t st)
     return (int)hipGetLastError();
 }
 
-int launch_empty(hipStream_t st) {
-    hipLaunchKernelGGL(dev::empty_kernel, dim3(1), dim3(64), 0, st);
+int launch_empty(hipStream_t st, int groups, int threads) {
+    hipLaunchKernelGGL(dev::empty_kernel, dim3(groups), dim3(threads), 0, st);
     return (int)hipGetLastError();
 }
 

@@ -138,7 +138,7 @@ int launch_bcsr(const DevBcsr& m, const float* x, float* y, hipStream_t st);
 int launch_bsr(const DevBsr& m, const float* x, float* y, hipStream_t st);
 int launch_xtile(const DevXtile& m, const float* x, float* y, hipStream_t st);
 int launch_copy_probe(const float4* src, float4* dst, size_t n4, hipStream_t st);
-int launch_empty(hipStream_t st);
+int launch_empty(hipStream_t st, int groups = 1, int threads = 64);
 
 // Constants of the adaptive CSR and COO kernels, shared with the host
 // planners that build their work lists.

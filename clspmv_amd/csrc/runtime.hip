@@ -178,6 +178,17 @@ int ell_rows(const clspmv_ell_matrix* m, RowLists* o) {
 // and COO parts.
 constexpr int kMethodXtile = 4;
 
+// Info of a part in the column-sliced layout (method 2); the long-row
+// carry buffer is scratch, not format storage.
+int finish_sliced(clspmv_part_t p, int fmt, int rows, int cols, long long nnz, const char* what) {
+    char name[48];
+    std::snprintf(name, sizeof(name), "%s_sliced_S%d", what, p->csrs.nslices);
+    fill_info(p, fmt, 2, rows, cols, nnz, name, p->csrs.nslices + (p->csrs.n_long ? 1 : 0));
+    p->info.stored_bytes -= p->carry_bytes;
+    p->info.alg_bytes -= p->carry_bytes;
+    return CLSPMV_OK;
+}
+
 int finish_xtile(clspmv_part_t p, int fmt, int rows, int cols, long long nnz) {
     char name[48];
     std::snprintf(name, sizeof(name), "xtile_B%d_G%d", p->xt.nblocks / p->xt.groups, p->xt.groups);
@@ -343,7 +354,7 @@ int clspmv_part_sell(clspmv_ctx_t ctx, const clspmv_sell_matrix* m, int method, 
                 }
         }
         int rc = 0;
-        if (method == kMethodXtile) {
+        if (method == kMethodXtile || method == 2) {
             // rows of the slices, padding slots included
             RowLists rl;
             rl.rp.assign((size_t)rows + 1, 0);
@@ -361,6 +372,11 @@ int clspmv_part_sell(clspmv_ctx_t ctx, const clspmv_sell_matrix* m, int method, 
                         rl.ci[(size_t)rl.rp[(size_t)r] + j] = m->sell_col_id[src];
                         rl.cv[(size_t)rl.rp[(size_t)r] + j] = m->sell_data[src];
                     }
+            }
+            if (method == 2) {
+                rc = build_csr_sliced(ctx, p, rl.rp.data(), rl.ci.data(), rl.cv.data(), rows, cols);
+                if (rc) return rc;
+                return finish_sliced(p, CLSPMV_FMT_SELL, rows, cols, m->matinfo.nnz, "sell");
             }
             if ((rc = build_xtile(ctx, p, rl.rp.data(), rl.ci.data(), rl.cv.data(), rows, cols))) return rc;
             return finish_xtile(p, CLSPMV_FMT_SELL, rows, cols, m->matinfo.nnz);
@@ -766,10 +782,15 @@ int clspmv_part_coo(clspmv_ctx_t ctx, const clspmv_coo_matrix* m, int method, cl
                 return CLSPMV_E_INVALID;
             }
         }
-        if (method == kMethodXtile) {
+        if (method == kMethodXtile || method == 2) {
             std::vector<int> rp((size_t)rows + 1, 0);
             for (int k = 0; k < nnz; ++k) rp[(size_t)m->coo_row_id[k] + 1]++;
             for (int r = 0; r < rows; ++r) rp[(size_t)r + 1] += rp[(size_t)r];
+            if (method == 2) {
+                int rc = build_csr_sliced(ctx, p, rp.data(), m->coo_col_id, m->coo_data, rows, cols);
+                if (rc) return rc;
+                return finish_sliced(p, CLSPMV_FMT_COO, rows, cols, nnz, "coo");
+            }
             int rc = build_xtile(ctx, p, rp.data(), m->coo_col_id, m->coo_data, rows, cols);
             if (rc) return rc;
             return finish_xtile(p, CLSPMV_FMT_COO, rows, cols, nnz);
@@ -1077,13 +1098,15 @@ int clspmv_part_launch(clspmv_part_t p, const float* x, float* y, hipStream_t st
         case CLSPMV_FMT_ELL:
             return p->info.method == 2 ? launch_csr_sliced(p->csrs, x, y, st)
                                        : launch_ell(p->ell, p->info.method, x, y, st);
-        case CLSPMV_FMT_SELL: return launch_sell(p->sell, x, y, st);
+        case CLSPMV_FMT_SELL:
+            return p->info.method == 2 ? launch_csr_sliced(p->csrs, x, y, st) : launch_sell(p->sell, x, y, st);
         case CLSPMV_FMT_CSR_SCALAR: return launch_csr_scalar(p->csr, x, y, st);
         case CLSPMV_FMT_CSR_VECTOR:
             return p->info.method == 0   ? launch_csr_vector(p->csr, p->csr_lanes, x, y, st)
                    : p->info.method == 2 ? launch_csr_sliced(p->csrs, x, y, st)
                                          : launch_csr_adaptive(p->csra, x, y, st);
-        case CLSPMV_FMT_COO: return launch_coo(p->coo, x, y, st);
+        case CLSPMV_FMT_COO:
+            return p->info.method == 2 ? launch_csr_sliced(p->csrs, x, y, st) : launch_coo(p->coo, x, y, st);
         case CLSPMV_FMT_DIA: return launch_dia(p->dia, p->info.method, x, y, st);
         case CLSPMV_FMT_BDIA: return launch_bdia(p->bdia, p->info.method, x, y, st);
         case CLSPMV_FMT_BELL: return launch_bell(p->bell, x, y, st);
@@ -1145,6 +1168,21 @@ int clspmv_probe_launch(clspmv_ctx_t ctx, int ntimes, float* usec) {
     launch_empty(st);
     (void)hipEventRecord(ctx->ev0, st);
     for (int i = 0; i < ntimes; ++i) launch_empty(st);
+    (void)hipEventRecord(ctx->ev1, st);
+    hipError_t he = hipEventSynchronize(ctx->ev1);
+    if (he != hipSuccess) return hip_status(he, "launch probe sync");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    *usec = ms * 1000.f / ntimes;
+    return CLSPMV_OK;
+}
+
+int clspmv_probe_launch_grid(clspmv_ctx_t ctx, int groups, int threads, int ntimes, float* usec) {
+    if (!ctx || !usec || ntimes <= 0 || groups <= 0 || threads <= 0 || threads > 1024) return CLSPMV_E_INVALID;
+    hipStream_t st = ctx->stream();
+    for (int i = 0; i < 10; ++i) launch_empty(st, groups, threads);
+    (void)hipEventRecord(ctx->ev0, st);
+    for (int i = 0; i < ntimes; ++i) launch_empty(st, groups, threads);
     (void)hipEventRecord(ctx->ev1, st);
     hipError_t he = hipEventSynchronize(ctx->ev1);
     if (he != hipSuccess) return hip_status(he, "launch probe sync");

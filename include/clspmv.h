@@ -312,10 +312,10 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
 /* Kernel variants (`method`) per part:
  *   ELL      0 one row per lane, 1 four rows per lane, 2 column-sliced,
  *            4 LDS x-tiled
- *   SELL     0 one row per lane, 4 LDS x-tiled
+ *   SELL     0 one row per lane, 2 column-sliced, 4 LDS x-tiled
  *   CSR vec  0 L lanes per row, 1 adaptive row blocks, 2 column-sliced,
  *            4 LDS x-tiled
- *   COO      0 segmented reduction, 4 LDS x-tiled
+ *   COO      0 segmented reduction, 2 column-sliced, 4 LDS x-tiled
  *   DIA      0 one row per lane, 1 four rows per lane
  *   BDIA     0 direct x reads, 1 LDS-staged x band
  *   BSR      0 sliced (64 block rows per slice), 1 (or L = 2..16) L lanes
@@ -345,6 +345,9 @@ int clspmv_part_free(clspmv_part_t p);
  * (A/src/mem_bandwidth.cpp:14-579, A/src/bench_overhead.cpp:14-184). */
 int clspmv_probe_copy(clspmv_ctx_t ctx, const void* d_src, void* d_dst, size_t bytes, int ntimes, float* gbps);
 int clspmv_probe_launch(clspmv_ctx_t ctx, int ntimes, float* usec_per_launch);
+/* Empty kernel of `groups` x `threads` launched ntimes back to back: the
+ * per-launch time of the overhead table (A/src/bench_overhead.cpp:117-165). */
+int clspmv_probe_launch_grid(clspmv_ctx_t ctx, int groups, int threads, int ntimes, float* usec_per_launch);
 
 #ifdef __cplusplus
 }

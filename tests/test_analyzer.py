@@ -87,3 +87,18 @@ def test_partition_conserves_nonzeros(bench):
 def test_missing_tables_is_an_io_error():
     b = C.c_void_p()
     assert L.lib().clspmv_bench_load(b"/nonexistent", C.byref(b)) == L.E_IO
+
+
+def test_long_rows_do_not_overflow_slot_totals(bench):
+    # 2M power-law rows up to 50,000 long: the reference's int sums of SELL
+    # slot totals overflow here (undefined); the restatement sums in 64 bits,
+    # so the SELL estimate stays large and the partition is a valid one.
+    from clspmv_amd import workloads as W
+    coo = W.make_coo("c1c")
+    mat = L.Cocktail()
+    L.lib().clspmv_init_cocktail(C.byref(mat))
+    L.check(L.lib().clspmv_analyze_matrix(bench, C.byref(coo.s), C.byref(mat), 1.0, 1, 0, 0), "analyze")
+    total = sum(getattr(mat, n).matinfo.nnz for n in ("bdia", "dia", "sbell", "bell", "bcsr", "sell", "ell", "csr", "coo")
+                if getattr(mat, f"ifuse{n}"))
+    assert total == coo.nnz
+    L.lib().clspmv_free_cocktail(C.byref(mat))

@@ -118,6 +118,11 @@ def test_column_sliced(ctx, small_slices, name, make):
     _check(coo, y, x, y0, f"CSR sliced {name} {info['kernel']}")
     y, info = gpu_spmv(ctx, L.FMT_ELL, M.coo2ell(coo), x, y0, 2)
     _check(coo, y, x, y0, f"ELL sliced {name} {info['kernel']}")
+    y, info = gpu_spmv(ctx, L.FMT_SELL, M.coo2sell(coo, 32), x, y0, 2)
+    _check(coo, y, x, y0, f"SELL sliced {name} {info['kernel']}")
+    y, info = gpu_spmv(ctx, L.FMT_COO, coo, x, y0, 2)
+    assert info["kernel"].startswith("coo_sliced"), info
+    _check(coo, y, x, y0, f"COO sliced {name} {info['kernel']}")
 
 
 def test_column_sliced_long_rows(ctx):
