@@ -108,6 +108,95 @@ int add_part(clspmv_gpumat_s* g, int which, const clspmv_cocktail* m) {
     return CLSPMV_OK;
 }
 
+// The flat parts (ELL, SELL, CSR, COO) of a partition are all row lists of
+// the same matrix.  On MI355X each of them is bound by its pass over x (a
+// gather or a staged sweep), so they run as one part: their rows are merged
+// (each row sorted by column) and laid out by the same rule as a single CSR
+// part.  ELL and SELL slots holding exactly 0.0 -- the padding of those
+// formats -- are left out (they add 0 * x[col], which changes nothing for
+// finite x).  The partition itself is unchanged.  Switched off with
+// clspmv_set_option("cocktail_merge_flat", 0).
+bool g_merge_flat = true;
+
+int merge_flat(clspmv_gpumat_s* g, const clspmv_cocktail* m) {
+    const int rows = m->mat_height, cols = m->mat_width;
+    std::vector<long long> cnt((size_t)rows + 1, 0);
+    const clspmv_ell_matrix& e = m->ell;
+    const clspmv_sell_matrix& sl = m->sell;
+    const clspmv_csr_matrix& c = m->csr;
+    const clspmv_coo_matrix& o = m->coo;
+    std::vector<int> coo_rp;
+    if (m->ifusecoo) {
+        coo_rp.assign((size_t)rows + 1, 0);
+        for (int k = 0; k < o.matinfo.nnz; ++k) coo_rp[(size_t)o.coo_row_id[k] + 1]++;
+        for (int r = 0; r < rows; ++r) coo_rp[(size_t)r + 1] += coo_rp[(size_t)r];
+    }
+    auto sell_width = [&](int r) {
+        const int S = (int)sl.sell_slice_height, s = r / S;
+        return (sl.sell_slice_ptr[s + 1] - sl.sell_slice_ptr[s]) / S;
+    };
+    auto ell_slot = [&](int r, int k) { return (size_t)k * e.ell_height_aligned + (size_t)r; };
+    auto sell_slot = [&](int r, int j) {
+        const int S = (int)sl.sell_slice_height;
+        return (size_t)sl.sell_slice_ptr[r / S] + (size_t)j * S + (size_t)(r % S);
+    };
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+    for (int r = 0; r < rows; ++r) {
+        long long n = 0;
+        if (m->ifuseell)
+            for (int k = 0; k < e.ell_num; ++k) n += e.ell_data[ell_slot(r, k)] != 0.0f;
+        if (m->ifusesell)
+            for (int j = 0, w = sell_width(r); j < w; ++j) n += sl.sell_data[sell_slot(r, j)] != 0.0f;
+        if (m->ifusecsr) n += c.csr_row_ptr[r + 1] - c.csr_row_ptr[r];
+        if (m->ifusecoo) n += coo_rp[(size_t)r + 1] - coo_rp[(size_t)r];
+        cnt[(size_t)r + 1] = n;
+    }
+    for (int r = 0; r < rows; ++r) cnt[(size_t)r + 1] += cnt[(size_t)r];
+    if (cnt[(size_t)rows] > INT32_MAX) return CLSPMV_E_TOO_LARGE;
+    std::vector<int> rp((size_t)rows + 1), ci((size_t)cnt[(size_t)rows]);
+    std::vector<float> cv((size_t)cnt[(size_t)rows]);
+    for (int r = 0; r <= rows; ++r) rp[(size_t)r] = (int)cnt[(size_t)r];
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(omp_threads())
+    for (int r = 0; r < rows; ++r) {
+        std::vector<std::pair<int, float>> row;
+        row.reserve((size_t)(rp[(size_t)r + 1] - rp[(size_t)r]));
+        if (m->ifuseell)
+            for (int k = 0; k < e.ell_num; ++k) {
+                const size_t i = ell_slot(r, k);
+                if (e.ell_data[i] != 0.0f) row.push_back({e.ell_col_id[i], e.ell_data[i]});
+            }
+        if (m->ifusesell)
+            for (int j = 0, w = sell_width(r); j < w; ++j) {
+                const size_t i = sell_slot(r, j);
+                if (sl.sell_data[i] != 0.0f) row.push_back({sl.sell_col_id[i], sl.sell_data[i]});
+            }
+        if (m->ifusecsr)
+            for (int k = c.csr_row_ptr[r]; k < c.csr_row_ptr[r + 1]; ++k) row.push_back({c.csr_col_id[k], c.csr_data[k]});
+        if (m->ifusecoo)
+            for (int k = coo_rp[(size_t)r]; k < coo_rp[(size_t)r + 1]; ++k) row.push_back({o.coo_col_id[k], o.coo_data[k]});
+        std::stable_sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (size_t k = 0; k < row.size(); ++k) {
+            ci[(size_t)rp[(size_t)r] + k] = row[k].first;
+            cv[(size_t)rp[(size_t)r] + k] = row[k].second;
+        }
+    }
+    clspmv_csr_matrix merged{};
+    merged.matinfo.width = cols;
+    merged.matinfo.height = rows;
+    merged.matinfo.nnz = (m->ifuseell ? e.matinfo.nnz : 0) + (m->ifusesell ? sl.matinfo.nnz : 0) +
+                         (m->ifusecsr ? c.matinfo.nnz : 0) + (m->ifusecoo ? o.matinfo.nnz : 0);
+    merged.csr_row_ptr = rp.data();
+    merged.csr_col_id = ci.data();
+    merged.csr_data = cv.data();
+    const long long stored = (long long)ci.size();
+    const int meth = !wide_x(cols) ? 1 : xtile_pays(rows, cols, stored) ? kXtile : 2;
+    clspmv_part_t p = nullptr;
+    int rc = clspmv_part_csr_vector(g->ctx, &merged, meth, &p);
+    if (rc) return rc;
+    g->parts.push_back(p);
+    return CLSPMV_OK;
+}
+
 int check_size(clspmv_gpumat_t g, int n, int limit) {
     if (!g || n < 0 || n > limit) {
         set_error("size %d outside [0, %d]", n, limit);
@@ -139,8 +228,11 @@ int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* m, const fl
     if (!rc && res) rc = clspmv_memcpy_h2d(ctx, g->y.ptr, res, (size_t)g->height * 4);
     const unsigned char use[9] = {m->ifusebdia, m->ifusedia, m->ifusesbell, m->ifusebell, m->ifusebcsr,
                                   m->ifusesell, m->ifuseell, m->ifusecsr, m->ifusecoo};
+    const int nflat = use[5] + use[6] + use[7] + use[8];
+    const bool merge = g_merge_flat && nflat >= 2;
     for (int i = 0; i < 9 && !rc; ++i)
-        if (use[i]) rc = add_part(g, i, m);
+        if (use[i] && !(merge && i >= 5)) rc = add_part(g, i, m);
+    if (!rc && merge) rc = merge_flat(g, m);
     if (!rc) rc = clspmv_synchronize(ctx);
     if (rc) {
         delete g;
@@ -204,6 +296,14 @@ int clspmv_gpumat_buffers(clspmv_gpumat_t g, void** d_vec, void** d_res) {
 }
 
 int clspmv_gpumat_num_parts(clspmv_gpumat_t g) { return g ? (int)g->parts.size() : 0; }
+
+__attribute__((visibility("hidden"))) int clspmv_cocktail_option(const char* name, long long value) {
+    if (name && std::strcmp(name, "cocktail_merge_flat") == 0) {
+        g_merge_flat = value != 0;
+        return CLSPMV_OK;
+    }
+    return CLSPMV_E_INVALID;
+}
 
 int clspmv_gpumat_part_info(clspmv_gpumat_t g, int i, clspmv_part_info* info) {
     if (!g || i < 0 || i >= (int)g->parts.size()) return CLSPMV_E_INVALID;

@@ -1068,8 +1068,11 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
     });
 }
 
+extern "C" int clspmv_cocktail_option(const char* name, long long value);  // cocktail.hip
+
 int clspmv_set_option(const char* name, long long value) {
     if (!name) return CLSPMV_E_INVALID;
+    if (std::strcmp(name, "cocktail_merge_flat") == 0) return clspmv_cocktail_option(name, value);
     if (std::strcmp(name, "xtile_max_groups") == 0) {
         if (value != 1 && value != 2 && value != 4 && value != 8) return CLSPMV_E_INVALID;
         g_xt_max_groups = (int)value;

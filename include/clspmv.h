@@ -335,7 +335,10 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  *                       256, one per CU);
  *   "xtile_max_groups"  1, 2, 4 or 8 column groups of the x-tiled layout
  *                       (default 1; > 1 assigns each XCD one group of x and
- *                       adds a partial-sum pass). */
+ *                       adds a partial-sum pass);
+ *   "cocktail_merge_flat"  1 (default): clspmv_init_mat_kernels runs the
+ *                       flat parts (ELL, SELL, CSR, COO) of a partition as
+ *                       one merged row-list part; 0: one part each. */
 int clspmv_set_option(const char* name, long long value);
 int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info);
 int clspmv_part_free(clspmv_part_t p);

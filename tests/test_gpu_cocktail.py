@@ -5,6 +5,7 @@ three parts; plus the y = A^k x iteration of A/src/eval.cpp:19-72."""
 import numpy as np
 import pytest
 
+from clspmv_amd import _lib as L
 from clspmv_amd import cocktail as CK
 from clspmv_amd import matrix as M
 from oracle import oracle as O
@@ -22,17 +23,40 @@ def tables():
     return CK.BenchTables(CK.REFERENCE_TABLES)
 
 
+@pytest.fixture(params=[1, 0], ids=["merged_flat", "separate_flat"])
+def merge_flat(request):
+    L.check(L.lib().clspmv_set_option(b"cocktail_merge_flat", request.param), "set_option")
+    yield request.param
+    L.check(L.lib().clspmv_set_option(b"cocktail_merge_flat", 1), "set_option")
+
+
 @pytest.mark.parametrize("case", CASES)
 @pytest.mark.parametrize("knobs", AC.KNOBS, ids=lambda k: f"m{k[0]}p{k[1]}f{k[2]}")
-def test_cocktail_spmv(ctx, tables, case, knobs):
+def test_cocktail_spmv(ctx, tables, merge_flat, case, knobs):
     coo = AC.make(GEN[case])
     mat = CK.analyze_matrix(coo.copy(), tables, knobs[0], bool(knobs[1]), bool(knobs[2]))
     x = M.gen_vector(coo.width, 21)
     y0 = M.gen_vector(coo.height, 22)
     gpu = CK.GpuCocktail(ctx, mat, x, y0)
+    flat = [n for n in mat.parts() if n in ("sell", "ell", "csr", "coo")]
+    expect = len(mat.parts()) - (len(flat) - 1 if merge_flat and len(flat) >= 2 else 0)
+    assert len(gpu.part_infos()) == expect, (mat.parts(), gpu.part_infos())
     gpu.do_spmv(1)
     y = gpu.result_to_cpu()
     rep = O.parity_report(y, coo, x, y0)
+    assert rep["ok"], (case, mat.parts(), rep)
+    gpu.free()
+
+
+@pytest.mark.parametrize("case", ["cocktail_64k", "band15_rand8", "zipf_20k"])
+def test_cocktail_mi355x_tables(ctx, merge_flat, case):
+    coo = AC.make(GEN[case])
+    mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(CK.MI355X_TABLES))
+    x = M.gen_vector(coo.width, 23)
+    y0 = M.gen_vector(coo.height, 24)
+    gpu = CK.GpuCocktail(ctx, mat, x, y0)
+    gpu.do_spmv(1)
+    rep = O.parity_report(gpu.result_to_cpu(), coo, x, y0)
     assert rep["ok"], (case, mat.parts(), rep)
     gpu.free()
 
