@@ -85,6 +85,10 @@ class Bsr(C.Structure):
                 ("bsr_row_ptr", _IP), ("bsr_col_id", _IP), ("bsr_data", _FP)]
 
 
+class SpmmTile(C.Structure):
+    _fields_ = [("rows_cap", C.c_int), ("nnz_cap", C.c_int), ("long_threshold", C.c_int)]
+
+
 class Cocktail(C.Structure):
     _fields_ = ([(f"ifuse{n}", C.c_ubyte) for n in ("bdia", "dia", "sbell", "bell", "bcsr", "sell", "ell", "csr", "coo")]
                 + [(f"{n}_meth_num", C.c_int) for n in ("bdia", "dia", "sbell", "bell", "bcsr", "sell", "ell", "csr",
@@ -197,6 +201,18 @@ _SIGS = [
     ("clspmv_gpumat_buffers", C.c_int, [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p)]),
     ("clspmv_gpumat_num_parts", C.c_int, [C.c_void_p]),
     ("clspmv_gpumat_part_info", C.c_int, [C.c_void_p, C.c_int, _P(PartInfo)]),
+    ("clspmv_spmm_default_tile", SpmmTile, [C.c_int]),
+    ("clspmv_batch_create", C.c_int, [C.c_void_p, C.c_int, _P(_P(Csr)), _P(C.c_int), _P(SpmmTile), _P(C.c_void_p)]),
+    ("clspmv_batch_free", C.c_int, [C.c_void_p]),
+    ("clspmv_batch_size", C.c_int, [C.c_void_p]),
+    ("clspmv_batch_buffers", C.c_int, [C.c_void_p, C.c_int, _P(C.c_void_p), _P(C.c_void_p)]),
+    ("clspmv_batch_set_x", C.c_int, [C.c_void_p, C.c_int, _FP]),
+    ("clspmv_batch_set_y", C.c_int, [C.c_void_p, C.c_int, _FP]),
+    ("clspmv_batch_get_y", C.c_int, [C.c_void_p, C.c_int, _FP]),
+    ("clspmv_batch_run", C.c_int, [C.c_void_p, C.c_int]),
+    ("clspmv_batch_capture", C.c_int, [C.c_void_p]),
+    ("clspmv_batch_stats", C.c_int, [C.c_void_p, _P(C.c_longlong), _P(C.c_longlong), _P(C.c_longlong),
+                                     _P(C.c_int), _P(C.c_int)]),
     ("clspmv_probe_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _FP]),
     ("clspmv_probe_launch", C.c_int, [C.c_void_p, C.c_int, _FP]),
     ("clspmv_probe_launch_grid", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _FP]),

@@ -123,6 +123,25 @@ struct DevXtile {
     int ntiles, nchunks, cols, rows;
 };
 
+// Batched multi-right-hand-side SpMV (k_spmm.hip).
+constexpr int kSpmmStageNnz = 512;    // entries of a STREAM item (staged in LDS)
+constexpr int kSpmmLongChunk = 1024;  // entries of a LONG item
+struct SpmmMat {
+    const int* row_ptr;
+    const int* col;
+    const float* val;
+    const float* x;  // cols x k, row-major
+    float* y;        // rows x k, row-major
+    int rows, k;     // k in {1, 4, 16}
+};
+struct DevSpmm {
+    const SpmmMat* mats;
+    const int4* items;   // STREAM {mat, r0, r1, 0} / LONG {mat, row, -(slot+1), chunk}
+    const int4* longs;   // {mat, row, first carry slot, chunks}
+    float* carry;        // 16 floats per LONG item
+    int n_items, n_long, grid;
+};
+
 int launch_ell(const DevEll& m, int method, const float* x, float* y, hipStream_t st);
 int launch_sell(const DevSell& m, const float* x, float* y, hipStream_t st);
 int launch_csr_scalar(const DevCsr& m, const float* x, float* y, hipStream_t st);
@@ -137,6 +156,7 @@ int launch_sbell(const DevSbell& m, const float* x, float* y, hipStream_t st);
 int launch_bcsr(const DevBcsr& m, const float* x, float* y, hipStream_t st);
 int launch_bsr(const DevBsr& m, const float* x, float* y, hipStream_t st);
 int launch_xtile(const DevXtile& m, const float* x, float* y, hipStream_t st);
+int launch_spmm(const DevSpmm& b, hipStream_t st);
 int launch_copy_probe(const float4* src, float4* dst, size_t n4, hipStream_t st);
 int launch_empty(hipStream_t st, int groups = 1, int threads = 64);
 
