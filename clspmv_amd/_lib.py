@@ -213,6 +213,15 @@ _SIGS = [
     ("clspmv_batch_capture", C.c_int, [C.c_void_p]),
     ("clspmv_batch_stats", C.c_int, [C.c_void_p, _P(C.c_longlong), _P(C.c_longlong), _P(C.c_longlong),
                                      _P(C.c_int), _P(C.c_int)]),
+    ("clspmv_mlp_create", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, _P(C.c_void_p)]),
+    ("clspmv_mlp_free", C.c_int, [C.c_void_p]),
+    ("clspmv_mlp_num_params", C.c_longlong, [C.c_void_p]),
+    ("clspmv_mlp_get_params", C.c_int, [C.c_void_p, _FP]),
+    ("clspmv_mlp_set_params", C.c_int, [C.c_void_p, _FP]),
+    ("clspmv_mlp_loss_grad", C.c_int, [C.c_void_p, _FP, _IP, C.c_int, _FP, _FP]),
+    ("clspmv_mlp_train", C.c_int, [C.c_void_p, _FP, _IP, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                   C.c_float, _FP]),
+    ("clspmv_mlp_predict", C.c_int, [C.c_void_p, _FP, C.c_int, _FP, _IP]),
     ("clspmv_probe_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _FP]),
     ("clspmv_probe_launch", C.c_int, [C.c_void_p, C.c_int, _FP]),
     ("clspmv_probe_launch_grid", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _FP]),

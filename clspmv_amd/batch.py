@@ -69,9 +69,9 @@ class Batch:
         return {"nnz": a.value, "nnz_rhs": b.value, "alg_bytes": c.value, "items": it.value, "grid": g.value}
 
     def free(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self.ctx.h:
             lib().clspmv_batch_free(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:

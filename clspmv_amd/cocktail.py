@@ -153,9 +153,9 @@ class GpuCocktail:
         return out
 
     def free(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self.ctx.h:
             lib().clspmv_free_mat_kernels(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:

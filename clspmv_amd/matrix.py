@@ -394,9 +394,10 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr:
+        # after Context.close() the device memory is gone with the context
+        if self.ptr and self.ctx.h:
             lib().clspmv_device_free(self.ctx.h, self.ptr)
-            self.ptr = C.c_void_p()
+        self.ptr = C.c_void_p()
 
     def __del__(self):
         try:
@@ -430,9 +431,9 @@ class Part:
         check(lib().clspmv_part_spmv(self.h, x.ptr, y.ptr, ntimes), "part_spmv")
 
     def free(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().clspmv_part_free(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:

@@ -1,0 +1,116 @@
+"""Trains the learned format selector on autotuner-timed matrices and
+reports its regret against exhaustive autotuning on held-out matrices.
+
+    python -m clspmv_amd.train_selector --n 4000 --out DIR
+
+DIR receives dataset.npz (features, candidate times, kinds), model.npz (the
+trained network and its feature normalisation) and report.json: per split the
+mean / median / p90 regret of the network's choice, of the fixed default
+(CSR adaptive row blocks) and -- on a subset of the held-out matrices -- of
+the Cocktail partition made with the MI355X tables, plus the share of
+matrices within 10 % of the best candidate.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import cocktail as CK
+from . import matrix as M
+from . import selector as S
+
+
+def log(s):
+    print(s, file=sys.stderr, flush=True)
+
+
+def summary(r):
+    r = np.asarray(r, np.float64)
+    return {"mean": float(np.mean(r)), "median": float(np.median(r)), "p90": float(np.percentile(r, 90)),
+            "within_10pct": float(np.mean(r <= 0.10)), "n": int(len(r))}
+
+
+def cocktail_times(ctx, idx, seed, log2_min, log2_max):
+    """Cocktail (MI355X tables, merged flat parts) time of matrices `idx` of
+    the dataset family."""
+    tables = CK.BenchTables(CK.MI355X_TABLES)
+    rng = np.random.default_rng(seed)
+    out = {}
+    want = set(int(i) for i in idx)
+    for i in range(max(want) + 1):
+        kind, coo = S.random_matrix(i, rng, log2_min, log2_max)
+        if i in want:
+            mat = CK.analyze_matrix(coo.copy(), tables)
+            gpu = CK.GpuCocktail(ctx, mat, M.gen_vector(coo.width, 1), np.zeros(coo.height, np.float32))
+            gpu.do_spmv(1)
+            ctx.sync()
+            ctx.time_begin()
+            gpu.do_spmv(5)
+            out[i] = ctx.time_end() / 5
+            gpu.free()
+            mat.free()
+        coo.free()
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=4000)
+    ap.add_argument("--test-frac", type=float, default=0.2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--log2-min", type=float, default=12)
+    ap.add_argument("--log2-max", type=float, default=18)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--cocktail-subset", type=int, default=200)
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args(argv)
+    os.makedirs(args.out, exist_ok=True)
+    ctx = M.Context(0)
+    t0 = time.perf_counter()
+    feats, times, kinds, nnz = S.build_dataset(ctx, args.n, args.seed, log, args.log2_min, args.log2_max)
+    np.savez(os.path.join(args.out, "dataset.npz"), features=feats, times=times, kinds=kinds, nnz=nnz,
+             candidates=np.array(S.CANDIDATE_NAMES))
+    log(f"[selector] dataset: {args.n} matrices in {time.perf_counter() - t0:.0f} s")
+    rng = np.random.default_rng(args.seed + 1)
+    perm = rng.permutation(args.n)
+    n_test = int(args.n * args.test_frac)
+    test, train = np.sort(perm[:n_test]), np.sort(perm[n_test:])
+    t1 = time.perf_counter()
+    sel = S.Selector.fit(ctx, feats[train], times[train], steps=args.steps)
+    log(f"[selector] trained in {time.perf_counter() - t1:.1f} s")
+    sel.save(os.path.join(args.out, "model.npz"))
+    report = {"n_train": int(len(train)), "n_test": int(len(test)), "candidates": S.CANDIDATE_NAMES,
+              "features": S.FEATURE_NAMES}
+    for name, idx in (("train", train), ("test", test)):
+        t = times[idx]
+        pick = sel.predict(feats[idx], t)
+        default = np.full(len(idx), S.DEFAULT)
+        report[name] = {
+            "network": summary(S.regret(t, pick)),
+            "default_csr_adaptive": summary(S.regret(t, default)),
+            "accuracy": float(np.mean(pick == np.argmin(t, axis=1))),
+            "best_counts": {S.CANDIDATE_NAMES[k]: int(v) for k, v in
+                            zip(*np.unique(np.argmin(t, axis=1), return_counts=True))},
+        }
+    sub = test[:args.cocktail_subset]
+    if len(sub):
+        ck = cocktail_times(ctx, sub, args.seed, args.log2_min, args.log2_max)
+        r = [ck[int(i)] / times[i].min() - 1.0 for i in sub]
+        report["test"]["cocktail_mi355x_tables"] = summary(r)
+        pick = sel.predict(feats[sub], times[sub])
+        report["test"]["network_on_cocktail_subset"] = summary(S.regret(times[sub], pick))
+    report["seconds"] = round(time.perf_counter() - t0, 1)
+    with open(os.path.join(args.out, "report.json"), "w") as f:
+        json.dump(report, f, indent=1)
+    print(json.dumps({k: report[k] for k in ("train", "test")}, indent=1), flush=True)
+    sel.net.free()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
