@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 import time
 
 import numpy as np
@@ -24,6 +25,8 @@ import numpy as np
 from . import _lib as L
 from . import matrix as M
 from ._lib import check, lib
+
+MODEL_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tables", "selector")
 
 # ------------------------------------------------------------------ network
 
@@ -326,6 +329,11 @@ class Selector:
 
     def save(self, path):
         np.savez(path, params=self.net.get_params(), mu=self.mu, sd=self.sd, hidden=self.net.h)
+
+    @classmethod
+    def load_default(cls, ctx):
+        """The network trained on MI355X timings (tables/selector/model.npz)."""
+        return cls.load(ctx, os.path.join(MODEL_DIR, "model.npz"))
 
     @classmethod
     def load(cls, ctx, path):

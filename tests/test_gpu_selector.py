@@ -36,3 +36,22 @@ def test_selector_learns_and_round_trips(ctx, dataset, tmp_path):
     assert np.array_equal(again.predict(feats, times), pick)
     fmt, meth, name = sel.choose(M.gen_laplacian2d(64, 64))
     assert name in S.CANDIDATE_NAMES
+
+
+def test_committed_model_regret_on_held_out_matrices(ctx):
+    # the network trained on MI355X timings, re-evaluated on the committed
+    # dataset's held-out split (same permutation as train_selector)
+    import json
+    import os
+    z = np.load(os.path.join(S.MODEL_DIR, "dataset.npz"), allow_pickle=False)
+    feats, times = z["features"], z["times"]
+    rep = json.load(open(os.path.join(S.MODEL_DIR, "report.json")))
+    n = len(feats)
+    perm = np.random.default_rng(1).permutation(n)
+    test = np.sort(perm[:rep["n_test"]])
+    sel = S.Selector.load_default(ctx)
+    r = S.regret(times[test], sel.predict(feats[test], times[test]))
+    assert abs(r.mean() - rep["test"]["network"]["mean"]) < 1e-6
+    assert r.mean() <= 0.10
+    default = S.regret(times[test], np.full(len(test), S.DEFAULT)).mean()
+    assert r.mean() < 0.5 * default
