@@ -142,6 +142,105 @@ def cocktail_c2(ctx, args):
     return out
 
 
+def batched_c4(ctx, args):
+    """The batch workload (c4): n matrices, k in {1, 4, 16}.  Three ways to
+    compute every Y_i += A_i X_i, timed with HIP events, in nonzeros x
+    right-hand sides per second:
+      plain     k_i single-vector FP32 SpMVs per matrix, CSR adaptive row
+                blocks (one launch each);
+      selector  the same with the format the learned selector picks;
+      batched   one replay of the captured graph (persistent kernel over all
+                matrices)."""
+    import numpy as np
+
+    from clspmv_amd import batch as B
+    from clspmv_amd import _lib as L
+    from clspmv_amd import matrix as M
+    from clspmv_amd import selector as S
+    from clspmv_amd import workloads as W
+
+    t0 = time.perf_counter()
+    mats = W.c4_matrices(args.c4_matrices)
+    csrs = [M.coo2csr(coo) for _, coo, _ in mats]
+    ks = [k for _, _, k in mats]
+    nnz = sum(coo.nnz for _, coo, _ in mats)
+    nnz_rhs = sum(coo.nnz * k for _, coo, k in mats)
+    log(f"c4: {len(mats)} matrices, {nnz} nnz, {nnz_rhs} nnz x rhs ({time.perf_counter() - t0:.1f}s setup)")
+    out = {"matrices": len(mats), "nnz": int(nnz), "nnz_x_rhs": int(nnz_rhs),
+           "rows_log2_range": [10, 20], "k_cycle": [1, 4, 16]}
+    reps = 3
+
+    # batched: one graph launch
+    b = B.Batch(ctx, csrs, ks)
+    for i, (_, coo, k) in enumerate(mats):
+        b.set_x(i, np.ones((coo.width, k), np.float32))
+    b.capture()
+    b.run(1)
+    ctx.sync()
+    ctx.time_begin()
+    b.run(reps)
+    ms_b = ctx.time_end() / reps
+    st = b.stats()
+    out["batched"] = {"ms": round(ms_b, 4), "eff_gnnz_per_s": round(nnz_rhs / (ms_b * 1e-3) / 1e9, 1),
+                      "alg_gbps": round(st["alg_bytes"] / (ms_b * 1e-3) / 1e9, 1), "items": st["items"],
+                      "launch": "one hipGraph replay"}
+    b.free()
+
+    def per_matrix(choose):
+        parts = []
+        for (kind, coo, k), csr in zip(mats, csrs):
+            fmt, meth, mat = choose(coo, csr)
+            part = M.Part(ctx, fmt, mat, meth)
+            if mat is not csr and mat is not coo:
+                mat.free()
+            x = M.DeviceArray.from_numpy(ctx, np.ones(coo.width, np.float32))
+            y = M.DeviceArray(ctx, coo.height)
+            parts.append((part, x, y, k))
+        for part, x, y, k in parts:
+            part.spmv(x, y, k)
+        ctx.sync()
+        ctx.time_begin()
+        for _ in range(reps):
+            for part, x, y, k in parts:
+                part.spmv(x, y, k)
+        ms = ctx.time_end() / reps
+        for part, x, y, _ in parts:
+            part.free()
+            x.free()
+            y.free()
+        return ms
+
+    ms_p = per_matrix(lambda coo, csr: (L.FMT_CSR_VECTOR, 1, csr))
+    out["plain"] = {"ms": round(ms_p, 4), "eff_gnnz_per_s": round(nnz_rhs / (ms_p * 1e-3) / 1e9, 1),
+                    "kernel": "csr_adaptive, k launches per matrix"}
+    sel = S.Selector.load_default(ctx)
+
+    def pick(coo, csr):
+        fmt, meth, name = sel.choose(coo)
+        key = S.CANDIDATES[S.CANDIDATE_NAMES.index(name)][3]
+        try:
+            mat = S._convert(key, coo) if key != "csr" else csr
+        except Exception:
+            mat = None
+        if mat is None:
+            return L.FMT_CSR_VECTOR, 1, csr
+        return fmt, meth, mat
+
+    ms_s = per_matrix(pick)
+    sel.net.free()
+    out["selector"] = {"ms": round(ms_s, 4), "eff_gnnz_per_s": round(nnz_rhs / (ms_s * 1e-3) / 1e9, 1),
+                       "kernel": "learned selector's format, k launches per matrix"}
+    out["batched_vs_plain"] = round(ms_p / ms_b, 3)
+    out["selector_vs_plain"] = round(ms_p / ms_s, 3)
+    for csr in csrs:
+        csr.free()
+    for _, coo, _ in mats:
+        coo.free()
+    log(f"c4: plain {ms_p:.2f} ms, selector {ms_s:.2f} ms, batched {ms_b:.2f} ms "
+        f"(batched {out['batched']['eff_gnnz_per_s']} G nnz*rhs/s, {out['batched_vs_plain']}x plain)")
+    return out
+
+
 def load_traffic(kernel):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -161,6 +260,8 @@ def main():
     ap.add_argument("--scale", type=int, default=1, help="divide matrix sizes (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the Cocktail composite leg")
+    ap.add_argument("--c4-matrices", type=int, default=256,
+                    help="matrices of the batched leg (c4 names 1024; 0 skips the leg)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported "
@@ -235,6 +336,7 @@ def main():
     }
 
     c2 = None if args.no_c2 else cocktail_c2(ctx, args)
+    c4 = batched_c4(ctx, args) if args.c4_matrices > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -276,6 +378,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cocktail_c2": c2,
+        "batched_c4": c4,
     }
     print(json.dumps(out), flush=True)
 

@@ -101,23 +101,36 @@ FEATURE_NAMES = [
 ]
 
 
+SAMPLE_NNZ = 1 << 20  # entries above which the locality features use a row sample
+
+
 def features(coo) -> np.ndarray:
-    """24 structural features of a COO matrix (rows sorted)."""
+    """24 structural features of a COO matrix (rows sorted).  Row-length
+    statistics use every row; the diagonal, block, segment and adjacency
+    features of a matrix with more than SAMPLE_NNZ entries are computed on
+    every s-th window of 48 rows (48 is a multiple of every block height),
+    s chosen so that about SAMPLE_NNZ entries remain."""
     r = coo.row_id.astype(np.int64)
     c = coo.col_id.astype(np.int64)
-    R, W, n = coo.height, coo.width, coo.nnz
+    R, W, n_all = coo.height, coo.width, coo.nnz
     lens = np.bincount(r, minlength=R).astype(np.float64)
-    mean = n / max(R, 1)
-    f = [math.log2(max(R, 1)), math.log2(max(W, 1)), math.log2(max(n, 1)), mean,
+    mean = n_all / max(R, 1)
+    n = n_all
+    f = [math.log2(max(R, 1)), math.log2(max(W, 1)), math.log2(max(n_all, 1)), mean,
          float(lens.std() / max(mean, 1e-9)), math.log2(max(lens.max(initial=0) / max(mean, 1e-9), 1e-9) + 1e-9)
-         if n else 0.0, float(np.mean(lens == 0)), float(np.mean(lens > 64)),
-         float(lens[lens > 64].sum() / max(n, 1)), math.log2(max(lens.max(initial=0) * R / max(n, 1), 1.0))]
+         if n_all else 0.0, float(np.mean(lens == 0)), float(np.mean(lens > 64)),
+         float(lens[lens > 64].sum() / max(n_all, 1)), math.log2(max(lens.max(initial=0) * R / max(n_all, 1), 1.0))]
+    if n_all > SAMPLE_NNZ:
+        stride = -(-n_all // SAMPLE_NNZ)
+        keep = (r // 48) % stride == 0
+        r, c = r[keep], c[keep]
+        n = len(r)
     if n:
         d = c - r
         _, cnt = np.unique(d, return_counts=True)
         cnt = np.sort(cnt)[::-1]
         f += [math.log2(len(cnt) + 1), float(cnt[:7].sum() / n), float(cnt[:27].sum() / n),
-              math.log2(max(len(cnt) * R / n, 1e-9)), float(np.mean(np.abs(d)) / max(W, 1)),
+              math.log2(max(len(cnt) * R / n_all, 1e-9)), float(np.mean(np.abs(d)) / max(W, 1)),
               float(np.mean(np.abs(d) <= 1024))]
         for b in (2, 3, 4, 8):
             nb = len(np.unique((r // b) * (W // b + 1) + c // b))

@@ -11,6 +11,8 @@ Configurations (all square, values N(0,1) unless stated, no duplicate
   c1d  1,048,576 block rows of dense 3x3 blocks, 9 distinct block
        columns per block row within +-5,000 of the diagonal           -> BSR 3x3
   c2   Cocktail composite, 4,194,304 rows (band + blocks + Zipf(1.6))
+  c4   batch: matrices drawn from the c1 generators with row counts
+       log-uniform in [2^10, 2^20], each with k in {1, 4, 16} right-hand sides
 Sizes can be scaled down (``scale``) for tests; the seeds are fixed.
 """
 from __future__ import annotations
@@ -70,3 +72,28 @@ def build(name: str, scale: int = 1) -> Workload:
     coo = make_coo(name, scale)
     fmt, meth, mat, desc = best_format(name, coo)
     return Workload(name, coo, fmt, meth, mat, desc)
+
+
+def c4_matrices(n: int = 1024, seed: int = 400, log2_min: float = 10, log2_max: float = 20):
+    """The batch workload: (kind, coo, k) for n seeded matrices from the c1
+    generators (27-point stencil, uniform 32 per row, Zipf(1.8) mean 16,
+    3x3 blocks x 9) at log-uniform row counts, k cycling 1, 4, 16."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    kinds = ["stencil27", "uniform", "zipf", "blocks"]
+    out = []
+    for i in range(n):
+        kind = kinds[i % 4]
+        rows = int(2 ** rng.uniform(log2_min, log2_max))
+        s = seed * 100003 + i
+        if kind == "stencil27":
+            coo = M.gen_stencil27(max(4, round(rows ** (1 / 3))), s)
+        elif kind == "uniform":
+            coo = M.gen_uniform_rows(rows, 32, s)
+        elif kind == "zipf":
+            coo = M.gen_zipf_rows(rows, 1.8, 16.0, min(50000, rows // 2), s)
+        else:
+            coo = M.gen_block_rows(max(16, rows // 3), 3, 9, 5000, s)
+        out.append((kind, coo, (1, 4, 16)[i % 3]))
+    return out
