@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libclspmv.so")
 
 OK = 0
+VAL_BF16X2, VAL_FP16_ROW, VAL_FP8_SLICE = 1, 2, 3
 E_INVALID, E_TOO_LARGE, E_PADDING, E_HIP, E_NOMEM, E_IO, E_SORT, E_UNSUPPORTED, E_NODEVICE = (
     -1, -2, -3, -4, -5, -6, -7, -8, -9)
 ERROR_NAMES = {0: "OK", -1: "E_INVALID", -2: "E_TOO_LARGE", -3: "E_PADDING", -4: "E_HIP",
@@ -201,6 +202,8 @@ _SIGS = [
     ("clspmv_gpumat_buffers", C.c_int, [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p)]),
     ("clspmv_gpumat_num_parts", C.c_int, [C.c_void_p]),
     ("clspmv_gpumat_part_info", C.c_int, [C.c_void_p, C.c_int, _P(PartInfo)]),
+    ("clspmv_part_dia_mixed", C.c_int, [C.c_void_p, _P(Dia), C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_bsr_mixed", C.c_int, [C.c_void_p, _P(Bsr), C.c_int, C.c_int, _P(C.c_void_p)]),
     ("clspmv_spmm_default_tile", SpmmTile, [C.c_int]),
     ("clspmv_batch_create", C.c_int, [C.c_void_p, C.c_int, _P(_P(Csr)), _P(C.c_int), _P(SpmmTile), _P(C.c_void_p)]),
     ("clspmv_batch_free", C.c_int, [C.c_void_p]),

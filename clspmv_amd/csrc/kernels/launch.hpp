@@ -123,6 +123,25 @@ struct DevXtile {
     int ntiles, nchunks, cols, rows;
 };
 
+// Mixed-precision parts (k_mixed.hip): value codecs, as CLSPMV_VAL_* in the ABI.
+constexpr int kCodecBf16x2 = 1;
+constexpr int kCodecFp16Row = 2;
+constexpr int kCodecFp8Slice = 3;
+struct DevDiaMixed {
+    const int* offsets;
+    const void* data;     // ndiag * L encoded values
+    const float* scale;   // FP16: L row scales; FP8: ndiag * nsl slice scales
+    int L, ndiag, nsl, rows, cols, codec;
+};
+struct DevBsrMixed {
+    const int* sp;        // slice pointer of the sliced layout
+    const void* col;      // int32 element columns, or int16 deltas
+    const int* base;      // per block row: column the first delta is added to
+    const void* data;     // bh*bw * stride encoded values
+    const float* scale;   // FP16: one per block row; FP8: bh*bw * nsl
+    int stride, brows, nsl, bh, bw, rows, cols, codec, delta;
+};
+
 // Batched multi-right-hand-side SpMV (k_spmm.hip).
 constexpr int kSpmmStageNnz = 512;    // entries of a STREAM item (staged in LDS)
 constexpr int kSpmmLongChunk = 1024;  // entries of a LONG item
@@ -157,6 +176,8 @@ int launch_bcsr(const DevBcsr& m, const float* x, float* y, hipStream_t st);
 int launch_bsr(const DevBsr& m, const float* x, float* y, hipStream_t st);
 int launch_xtile(const DevXtile& m, const float* x, float* y, hipStream_t st);
 int launch_spmm(const DevSpmm& b, hipStream_t st);
+int launch_dia_mixed(const DevDiaMixed& m, const float* x, float* y, hipStream_t st);
+int launch_bsr_mixed(const DevBsrMixed& m, const float* x, float* y, hipStream_t st);
 int launch_copy_probe(const float4* src, float4* dst, size_t n4, hipStream_t st);
 int launch_empty(hipStream_t st, int groups = 1, int threads = 64);
 

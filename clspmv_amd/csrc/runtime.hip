@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "clspmv.h"
+#include "host/codec.hpp"
 #include "host/internal.hpp"
 #include "kernels/launch.hpp"
 #include "runtime.hpp"
@@ -84,6 +85,8 @@ struct clspmv_part_s {
     DevBcsr bcsr{};
     DevBsr bsr{};
     DevXtile xt{};
+    DevDiaMixed diam{};
+    DevBsrMixed bsrm{};
 
     void* add(clspmv_ctx_t c, const void* host, size_t bytes, int* rc) {
         bufs.emplace_back();
@@ -1068,6 +1071,241 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
     });
 }
 
+// ---------------------------------------------------------------- mixed precision
+
+namespace {
+
+constexpr int kMethodMixed = 16;  // part method = 16 + codec (+ 4 with int16 delta columns)
+
+bool codec_ok(int codec) { return codec == kCodecBf16x2 || codec == kCodecFp16Row || codec == kCodecFp8Slice; }
+
+// Encodes n values v[i * step] (i < n) with `codec` into out (bytes per
+// value: 4, 2, 1); FP16 divides by `s` first, FP8 by its own scale.
+uint32_t bf16x2_word(float v) {
+    const uint16_t hi = f32_to_bf16(v);
+    const uint16_t lo = f32_to_bf16(v - bf16_to_f32(hi));
+    return ((uint32_t)hi << 16) | lo;
+}
+
+float fp8_scale(float amax) { return amax > 0.f ? amax / 448.0f : 1.0f; }
+
+const char* codec_name(int codec) {
+    return codec == kCodecBf16x2 ? "bf16x2" : codec == kCodecFp16Row ? "fp16row" : "fp8slice";
+}
+
+}  // namespace
+
+extern "C" {
+
+int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int codec, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        if (!codec_ok(codec)) return CLSPMV_E_INVALID;
+        const int rows = m->matinfo.height, cols = m->matinfo.width, L = m->dia_length_aligned, nd = m->dia_num;
+        if (rows < 0 || L < rows || nd < 0 || (L % 64) != 0) {
+            set_error("mixed DIA needs dia_length_aligned a multiple of 64 (coo2dia alignment 64)");
+            return CLSPMV_E_INVALID;
+        }
+        const size_t n = (size_t)nd * L;
+        const int nsl = L / 64;
+        std::vector<float> scale;
+        std::vector<unsigned char> enc;
+        if (codec == kCodecBf16x2) {
+            std::vector<uint32_t> w(n);
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+            for (long long i = 0; i < (long long)n; ++i) w[(size_t)i] = bf16x2_word(m->dia_data[i]);
+            enc.assign(reinterpret_cast<unsigned char*>(w.data()), reinterpret_cast<unsigned char*>(w.data()) + n * 4);
+        } else if (codec == kCodecFp16Row) {
+            scale.assign((size_t)L, 1.0f);
+            for (int r = 0; r < L; ++r) {
+                float a = 0.f;
+                for (int d = 0; d < nd; ++d) a = std::max(a, std::fabs(m->dia_data[(size_t)d * L + r]));
+                if (a > 0.f) scale[(size_t)r] = a;
+            }
+            std::vector<uint16_t> h(n);
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+            for (long long i = 0; i < (long long)n; ++i)
+                h[(size_t)i] = f32_to_f16(m->dia_data[i] / scale[(size_t)(i % L)]);
+            enc.assign(reinterpret_cast<unsigned char*>(h.data()), reinterpret_cast<unsigned char*>(h.data()) + n * 2);
+        } else {
+            scale.assign((size_t)nd * nsl, 1.0f);
+            enc.assign(n, 0);
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+            for (long long q = 0; q < (long long)nd * nsl; ++q) {
+                const size_t d = (size_t)q / nsl, s = (size_t)q % nsl;
+                const float* v = m->dia_data + d * L + s * 64;
+                float a = 0.f;
+                for (int i = 0; i < 64; ++i) a = std::max(a, std::fabs(v[i]));
+                const float sc = fp8_scale(a);
+                scale[(size_t)q] = sc;
+                for (int i = 0; i < 64; ++i) enc[d * L + s * 64 + (size_t)i] = f32_to_e4m3(v[i] / sc);
+            }
+        }
+        int rc = 0;
+        DevDiaMixed d{};
+        d.offsets = (const int*)p->add(ctx, m->dia_offsets, (size_t)std::max(1, nd) * 4, &rc);
+        if (rc) return rc;
+        d.data = p->add(ctx, enc.data(), std::max<size_t>(16, enc.size()), &rc);
+        if (rc) return rc;
+        if (!scale.empty()) {
+            d.scale = (const float*)p->add(ctx, scale.data(), scale.size() * 4, &rc);
+            if (rc) return rc;
+        }
+        d.L = L;
+        d.ndiag = nd;
+        d.nsl = nsl;
+        d.rows = rows;
+        d.cols = cols;
+        d.codec = codec;
+        p->diam = d;
+        char name[40];
+        std::snprintf(name, sizeof(name), "dia_mixed_%s", codec_name(codec));
+        fill_info(p, CLSPMV_FMT_DIA, kMethodMixed + codec, rows, cols, m->matinfo.nnz, name, 1);
+        return CLSPMV_OK;
+    });
+}
+
+int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int codec, int delta_cols,
+                          clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        if (!codec_ok(codec)) return CLSPMV_E_INVALID;
+        const unsigned bw = m->bsr_bwidth, bh = m->bsr_bheight;
+        if (bw != bh || bw < 2 || bw > 4) return CLSPMV_E_UNSUPPORTED;
+        const int rows = m->matinfo.height, cols = m->matinfo.width;
+        const int brows = m->bsr_row_num, nb = m->bsr_block_num, stride = m->bsr_block_stride;
+        if (brows != (rows + (int)bh - 1) / (int)bh || stride < nb || m->bsr_row_ptr[0] != 0 ||
+            m->bsr_row_ptr[brows] != nb)
+            return CLSPMV_E_INVALID;
+        for (int r = 0; r < brows; ++r)
+            if (m->bsr_row_ptr[r + 1] < m->bsr_row_ptr[r]) return CLSPMV_E_INVALID;
+        if (!cols_ok(m->bsr_col_id, (size_t)nb, 0, std::max(1, cols))) {
+            set_error("BSR block column out of range");
+            return CLSPMV_E_INVALID;
+        }
+        const unsigned bs = bw * bh;
+        const int S = kBsrSlice, ns = (brows + S - 1) / S;
+        std::vector<int> sp((size_t)ns + 1, 0);
+        for (int s = 0; s < ns; ++s) {
+            int wmax = 0;
+            for (int r = s * S; r < std::min(brows, s * S + S); ++r)
+                wmax = std::max(wmax, m->bsr_row_ptr[r + 1] - m->bsr_row_ptr[r]);
+            if ((int64_t)sp[s] + (int64_t)wmax * S > INT32_MAX) return CLSPMV_E_TOO_LARGE;
+            sp[s + 1] = sp[s] + wmax * S;
+        }
+        const size_t slots = (size_t)sp[ns], sstride = std::max<size_t>(64, slots);
+        std::vector<int> scol(slots, 0), base((size_t)std::max(1, brows), 0);
+        std::vector<float> sdat(sstride * bs, 0.0f);
+        for (int r = 0; r < brows; ++r) {
+            const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
+            const int a = m->bsr_row_ptr[r], e = m->bsr_row_ptr[r + 1];
+            int last = a < e ? m->bsr_col_id[a] : 0;
+            base[(size_t)r] = last;
+            for (int k = 0; k < width; ++k) {
+                const size_t idx = (size_t)sp[s] + (size_t)k * S + t;
+                if (a + k < e) {
+                    last = m->bsr_col_id[a + k];
+                    for (unsigned q = 0; q < bs; ++q) sdat[(size_t)q * sstride + idx] = m->bsr_data[(size_t)q * stride + a + k];
+                }
+                scol[idx] = last;
+            }
+        }
+        // int16 deltas when every step fits, else int32 columns
+        bool delta = delta_cols != 0;
+        std::vector<int16_t> dcol;
+        if (delta) {
+            dcol.assign(slots, 0);
+            for (int r = 0; r < brows && delta; ++r) {
+                const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
+                int prev = base[(size_t)r];
+                for (int k = 0; k < width; ++k) {
+                    const size_t idx = (size_t)sp[s] + (size_t)k * S + t;
+                    const int dlt = scol[idx] - prev;
+                    if (dlt < INT16_MIN || dlt > INT16_MAX) {
+                        delta = false;
+                        break;
+                    }
+                    dcol[idx] = (int16_t)dlt;
+                    prev = scol[idx];
+                }
+            }
+        }
+        const int nsl = ns;
+        std::vector<float> scale;
+        std::vector<unsigned char> enc;
+        const size_t nv = sstride * bs;
+        if (codec == kCodecBf16x2) {
+            std::vector<uint32_t> w(nv);
+            for (size_t i = 0; i < nv; ++i) w[i] = bf16x2_word(sdat[i]);
+            enc.assign(reinterpret_cast<unsigned char*>(w.data()), reinterpret_cast<unsigned char*>(w.data()) + nv * 4);
+        } else if (codec == kCodecFp16Row) {
+            scale.assign((size_t)std::max(1, brows), 1.0f);
+            for (int r = 0; r < brows; ++r) {
+                const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
+                float amax = 0.f;
+                for (int k = 0; k < width; ++k)
+                    for (unsigned q = 0; q < bs; ++q)
+                        amax = std::max(amax, std::fabs(sdat[(size_t)q * sstride + (size_t)sp[s] + (size_t)k * S + t]));
+                if (amax > 0.f) scale[(size_t)r] = amax;
+            }
+            std::vector<uint16_t> h(nv, 0);
+            for (int r = 0; r < brows; ++r) {
+                const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
+                for (int k = 0; k < width; ++k)
+                    for (unsigned q = 0; q < bs; ++q) {
+                        const size_t i = (size_t)q * sstride + (size_t)sp[s] + (size_t)k * S + t;
+                        h[i] = f32_to_f16(sdat[i] / scale[(size_t)r]);
+                    }
+            }
+            enc.assign(reinterpret_cast<unsigned char*>(h.data()), reinterpret_cast<unsigned char*>(h.data()) + nv * 2);
+        } else {
+            scale.assign((size_t)bs * nsl, 1.0f);
+            enc.assign(nv, 0);
+            for (unsigned q = 0; q < bs; ++q)
+                for (int s = 0; s < ns; ++s) {
+                    float amax = 0.f;
+                    for (int i = sp[s]; i < sp[s + 1]; ++i) amax = std::max(amax, std::fabs(sdat[(size_t)q * sstride + i]));
+                    const float sc = fp8_scale(amax);
+                    scale[(size_t)q * nsl + s] = sc;
+                    for (int i = sp[s]; i < sp[s + 1]; ++i)
+                        enc[(size_t)q * sstride + i] = f32_to_e4m3(sdat[(size_t)q * sstride + i] / sc);
+                }
+        }
+        int rc = 0;
+        DevBsrMixed d{};
+        d.sp = (const int*)p->add(ctx, sp.data(), sp.size() * 4, &rc);
+        if (rc) return rc;
+        if (delta) {
+            d.col = p->add(ctx, dcol.data(), std::max<size_t>(16, slots * 2), &rc);
+            if (rc) return rc;
+            d.base = (const int*)p->add(ctx, base.data(), base.size() * 4, &rc);
+        } else {
+            d.col = p->add(ctx, scol.data(), std::max<size_t>(16, slots * 4), &rc);
+        }
+        if (rc) return rc;
+        d.data = p->add(ctx, enc.data(), std::max<size_t>(16, enc.size()), &rc);
+        if (rc) return rc;
+        if (!scale.empty()) {
+            d.scale = (const float*)p->add(ctx, scale.data(), scale.size() * 4, &rc);
+            if (rc) return rc;
+        }
+        d.stride = (int)sstride;
+        d.brows = brows;
+        d.nsl = nsl;
+        d.bh = (int)bh;
+        d.bw = (int)bw;
+        d.rows = rows;
+        d.cols = cols;
+        d.codec = codec;
+        d.delta = delta ? 1 : 0;
+        p->bsrm = d;
+        char name[48];
+        std::snprintf(name, sizeof(name), "bsr%u%u_mixed_%s%s", bh, bw, codec_name(codec), delta ? "_d16" : "");
+        fill_info(p, CLSPMV_FMT_BSR, kMethodMixed + codec + (delta ? 4 : 0), rows, cols, m->matinfo.nnz, name, 1);
+        return CLSPMV_OK;
+    });
+}
+
+}  // extern "C"
+
 extern "C" int clspmv_cocktail_option(const char* name, long long value);  // cocktail.hip
 
 int clspmv_set_option(const char* name, long long value) {
@@ -1093,6 +1331,11 @@ int clspmv_set_option(const char* name, long long value) {
 }
 
 int clspmv_part_launch(clspmv_part_t p, const float* x, float* y, hipStream_t st) {
+    if (p->info.method >= kMethodMixed) {
+        if (p->info.format == CLSPMV_FMT_DIA) return launch_dia_mixed(p->diam, x, y, st);
+        if (p->info.format == CLSPMV_FMT_BSR) return launch_bsr_mixed(p->bsrm, x, y, st);
+        return (int)hipErrorInvalidValue;
+    }
     if (p->info.method == kMethodXtile &&
         (p->info.format == CLSPMV_FMT_ELL || p->info.format == CLSPMV_FMT_SELL ||
          p->info.format == CLSPMV_FMT_CSR_VECTOR || p->info.format == CLSPMV_FMT_COO))

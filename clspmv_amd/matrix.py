@@ -440,3 +440,23 @@ class Part:
             self.free()
         except Exception:
             pass
+
+
+class MixedPart(Part):
+    """A mixed-precision part (clspmv_part_dia_mixed / clspmv_part_bsr_mixed):
+    values stored with `codec` (L.VAL_*), FP32 accumulation."""
+
+    def __init__(self, ctx: Context, fmt: int, mat, codec: int, delta_cols: bool = True):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        if fmt == L.FMT_DIA:
+            check(lib().clspmv_part_dia_mixed(ctx.h, C.byref(mat.s), codec, C.byref(self.h)), "part_dia_mixed")
+        elif fmt == L.FMT_BSR:
+            check(lib().clspmv_part_bsr_mixed(ctx.h, C.byref(mat.s), codec, int(delta_cols), C.byref(self.h)),
+                  "part_bsr_mixed")
+        else:
+            raise ValueError("mixed precision is provided for DIA and BSR parts")
+        inf = L.PartInfo()
+        check(lib().clspmv_part_get_info(self.h, C.byref(inf)), "part_get_info")
+        self.info = {k: getattr(inf, k) for k, _ in L.PartInfo._fields_}
+        self.info["kernel"] = inf.kernel.decode()
