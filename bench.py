@@ -12,11 +12,16 @@ value = sum(2*nnz) / step time (GFLOP/s).  Each kernel is also timed alone
 with HIP events on the context stream; the dominant one is reported against
 the HBM roofline with its algorithmic bytes (format arrays + x + y read and
 write).  The Cocktail composite (c2, 4M rows) is then analyzed, uploaded and
-timed beside the best single format (extra field "cocktail_c2").  The
+timed beside the best single format (extra field "cocktail_c2"), the c1
+matrices are run again with mixed-precision values (field "mixed_c3": time,
+bytes, speedup and error against an FP64 product per codec), and a batch of
+matrices with 1-16 right-hand sides goes through one captured graph (field
+"batched_c4").  The
 reference's CPU SpMV (spmv_only -> coo_spmv, one core) is timed on the host
 on a bounded sample of the same c1 matrices.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-c2] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-c2] [--no-c3] [--no-cpu]
+                       [--c4-matrices N]
 """
 from __future__ import annotations
 
@@ -142,6 +147,88 @@ def cocktail_c2(ctx, args):
     return out
 
 
+def _fp64_reference(coo, x):
+    """FP64 y = A x and the row sums S = |A| |x| (scipy, on the host)."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    a = sp.csr_matrix((coo.data.astype(np.float64), (coo.row_id, coo.col_id)), shape=(coo.height, coo.width))
+    x64 = x.astype(np.float64)
+    y64 = a @ x64
+    a.data = np.abs(a.data)
+    return y64, a @ np.abs(x64)
+
+
+def _errors(y, y64, s):
+    """max_i |y_i - y64_i| / max(|y64_i|, S_i) (componentwise, S = |A||x|) and
+    ||y - y64||_2 / ||y64||_2."""
+    import numpy as np
+
+    d = np.abs(y.astype(np.float64) - y64)
+    return (float(np.max(d / np.maximum(np.maximum(np.abs(y64), s), 1e-300))),
+            float(np.linalg.norm(d) / max(np.linalg.norm(y64), 1e-300)))
+
+
+def mixed_c3(ctx, args, work, per):
+    """Mixed precision on the c1 matrices (c3): each matrix in its c1 layout
+    with values stored as BF16 hi+lo, FP16 with a row scale or FP8 E4M3 with
+    a slice scale (DIA and x-tiled CSR parts; BSR with int16 delta block
+    columns), FP32 accumulation.  Per matrix and codec: kernel time (HIP
+    events), stored bytes, effective nonzeros/s, speedup over the FP32 part,
+    and the error of one y = A x against the FP64 product."""
+    import numpy as np
+
+    from clspmv_amd import _lib as L
+    from clspmv_amd import matrix as M
+
+    k = max(10, args.steps)
+    out = {}
+
+    def run_once(part, dx, n):
+        dy = M.DeviceArray.from_numpy(ctx, np.zeros(n, np.float32))
+        part.spmv(dx, dy, 1)
+        y = dy.download()
+        dy.free()
+        return y
+
+    def time_part(part, dx, dy):
+        part.spmv(dx, dy, 2)
+        ctx.sync()
+        ctx.time_begin()
+        part.spmv(dx, dy, k)
+        return ctx.time_end() / k
+
+    for w in work:
+        coo, name = w["coo"], w["name"]
+        y64, srow = _fp64_reference(coo, w["x"])
+        e_max, e_norm = _errors(run_once(w["part"], w["dx"], coo.height), y64, srow)
+        ms32 = per[name]["ms"]
+        rec = {"fp32": {"kernel": w["part"].info["kernel"], "ms": ms32,
+                        "stored_bytes": int(w["part"].info["stored_bytes"]),
+                        "eff_gnnz_per_s": round(coo.nnz / (ms32 * 1e-3) / 1e9, 1),
+                        "max_rel_err": e_max, "norm_rel_err": e_norm}}
+        if name == "c1a":
+            fmt, mat = L.FMT_DIA, M.coo2dia(coo, 256)
+        elif name == "c1d":
+            fmt, mat = L.FMT_BSR, M.coo2bsr(coo, 3, 3)
+        else:
+            fmt, mat = L.FMT_CSR_VECTOR, M.coo2csr(coo)
+        for codec, label in ((L.VAL_BF16X2, "bf16x2"), (L.VAL_FP16_ROW, "fp16row"), (L.VAL_FP8_SLICE, "fp8slice")):
+            part = M.MixedPart(ctx, fmt, mat, codec, True)
+            ms = time_part(part, w["dx"], w["dy"])
+            e_max, e_norm = _errors(run_once(part, w["dx"], coo.height), y64, srow)
+            rec[label] = {"kernel": part.info["kernel"], "ms": round(ms, 5),
+                          "stored_bytes": int(part.info["stored_bytes"]),
+                          "eff_gnnz_per_s": round(coo.nnz / (ms * 1e-3) / 1e9, 1),
+                          "speedup_vs_fp32": round(ms32 / ms, 3), "max_rel_err": e_max, "norm_rel_err": e_norm}
+            log(f"c3 {name} {part.info['kernel']}: {ms * 1e3:.1f} us ({ms32 / ms:.2f}x FP32), "
+                f"{part.info['stored_bytes'] / 1e6:.1f} MB, max rel err {e_max:.2e}, norm rel err {e_norm:.2e}")
+            part.free()
+        mat.free()
+        out[name] = rec
+    return out
+
+
 def batched_c4(ctx, args):
     """The batch workload (c4): n matrices, k in {1, 4, 16}.  Three ways to
     compute every Y_i += A_i X_i, timed with HIP events, in nonzeros x
@@ -260,6 +347,7 @@ def main():
     ap.add_argument("--scale", type=int, default=1, help="divide matrix sizes (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the Cocktail composite leg")
+    ap.add_argument("--no-c3", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--c4-matrices", type=int, default=256,
                     help="matrices of the batched leg (c4 names 1024; 0 skips the leg)")
     args = ap.parse_args()
@@ -335,6 +423,7 @@ def main():
         "kernel": d["kernel"], "matrix": dom, "alg_bytes_per_launch": d["alg_bytes"],
     }
 
+    c3 = None if args.no_c3 else mixed_c3(ctx, args, work, per)
     c2 = None if args.no_c2 else cocktail_c2(ctx, args)
     c4 = batched_c4(ctx, args) if args.c4_matrices > 0 else None
 
@@ -377,6 +466,7 @@ def main():
         "per_matrix": per,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed_c3": c3,
         "cocktail_c2": c2,
         "batched_c4": c4,
     }

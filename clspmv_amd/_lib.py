@@ -204,6 +204,7 @@ _SIGS = [
     ("clspmv_gpumat_part_info", C.c_int, [C.c_void_p, C.c_int, _P(PartInfo)]),
     ("clspmv_part_dia_mixed", C.c_int, [C.c_void_p, _P(Dia), C.c_int, _P(C.c_void_p)]),
     ("clspmv_part_bsr_mixed", C.c_int, [C.c_void_p, _P(Bsr), C.c_int, C.c_int, _P(C.c_void_p)]),
+    ("clspmv_part_csr_mixed", C.c_int, [C.c_void_p, _P(Csr), C.c_int, _P(C.c_void_p)]),
     ("clspmv_spmm_default_tile", SpmmTile, [C.c_int]),
     ("clspmv_batch_create", C.c_int, [C.c_void_p, C.c_int, _P(_P(Csr)), _P(C.c_int), _P(SpmmTile), _P(C.c_void_p)]),
     ("clspmv_batch_free", C.c_int, [C.c_void_p]),

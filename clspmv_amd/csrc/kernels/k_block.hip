@@ -170,11 +170,15 @@ __global__ __launch_bounds__(256) void bsr_kernel(const int* __restrict__ rp, co
 // of (slice s, block row t, k) is sp[s] + k*64 + t, so lane t's loads of the
 // column and of each of the BH*BW value streams are 256 contiguous bytes per
 // wave.  Padding slots repeat the row's last column with zero values.
+// Blocks go G = bsr_group(BH * BW) at a time: their column loads, then all
+// x and value loads of the G blocks, are issued back to back (x through
+// load_xblock: no load sits behind a branch).  Requires cols >= BW.
 template <int BH, int BW>
 __global__ __launch_bounds__(256) void bsr_sliced_kernel(const int* __restrict__ sp, const int* __restrict__ col,
                                                          const float* __restrict__ data, int stride, int brows,
                                                          const float* __restrict__ x, float* __restrict__ y,
                                                          int rows, int cols) {
+    constexpr int G = bsr_group(BH * BW);
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= brows) return;
     const int s = r >> 6, t = r & 63;
@@ -183,17 +187,29 @@ __global__ __launch_bounds__(256) void bsr_sliced_kernel(const int* __restrict__
     float acc[BH];
 #pragma unroll
     for (int h = 0; h < BH; ++h) acc[h] = 0.f;
-    for (int k = 0; k < width; ++k) {
+    int k = 0;
+    for (; k + G <= width; k += G) {
+        int c[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) c[u] = ld_stream(col + b + ((k + u) << 6) + t);
+        float xv[G][BW], v[G][BH * BW];
+#pragma unroll
+        for (int u = 0; u < G; ++u) load_xblock<BW>(x, c[u], cols, xv[u]);
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+#pragma unroll
+            for (int e = 0; e < BH * BW; ++e) v[u][e] = ld_stream(data + (size_t)e * stride + b + ((k + u) << 6) + t);
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+#pragma unroll
+            for (int h = 0; h < BH; ++h)
+#pragma unroll
+                for (int w = 0; w < BW; ++w) acc[h] = fmaf(v[u][h * BW + w], xv[u][w], acc[h]);
+    }
+    for (; k < width; ++k) {
         const int idx = b + (k << 6) + t;
-        const int c = ld_stream(col + idx);
         float xv[BW];
-        if (c + BW <= cols) {
-#pragma unroll
-            for (int w = 0; w < BW; ++w) xv[w] = x[c + w];
-        } else {
-#pragma unroll
-            for (int w = 0; w < BW; ++w) xv[w] = (c + w < cols) ? x[c + w] : 0.f;
-        }
+        load_xblock<BW>(x, ld_stream(col + idx), cols, xv);
 #pragma unroll
         for (int h = 0; h < BH; ++h)
 #pragma unroll

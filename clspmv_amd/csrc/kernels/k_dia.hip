@@ -5,7 +5,8 @@
 // read with a bounds test instead of the reference's zero-padded copy
 // (vec_extended, A/src/spmv_cocktail.cpp:253-301).
 //   DIA  method 0: one row per lane (A/kernels/spmv_dia.cl:4-28 "gpu_dia");
-//        method 1: four rows per lane, dwordx4 data loads ("gpu_dia_v4").
+//        method 1: four rows per lane, dwordx4 data and x loads, a
+//                  branch-free loop for interior workgroups ("gpu_dia_v4").
 //   BDIA method 0: one row per lane, diagonals of a band read x directly
 //                  (A/kernels/spmv_bdia.cl:46-79 "gpu_bdia_nlvec");
 //        method 1: the x window of each band (256 + width - 1 values) is
@@ -46,34 +47,60 @@ __global__ __launch_bounds__(256) void dia_row_kernel(const int* __restrict__ of
     y[row] = acc;
 }
 
+// Four consecutive rows per lane: one dwordx4 value load and one dwordx4
+// x load (x[c .. c+3], c = row + offset, four consecutive columns) per
+// diagonal.  A workgroup covers 1024 rows; when every column it can touch
+// (rows row0 .. row0+1023 plus the smallest and largest offset) lies inside
+// x -- all but the first and last few workgroups of a banded matrix --
+// it takes the branch-free interior loop, chosen once per workgroup.
 __global__ __launch_bounds__(256) void dia_row4_kernel(const int* __restrict__ offsets,
                                                        const float4* __restrict__ data, int L4, int ndiag,
-                                                       const float* __restrict__ x, float* __restrict__ y, int rows,
-                                                       int cols) {
+                                                       int offmin, int offmax, const float* __restrict__ x,
+                                                       float* __restrict__ y, int rows, int cols) {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int r4 = tile * 256 + threadIdx.x;
     const int row = r4 * 4;
+    const long long wrow0 = (long long)tile * 1024;
+    const bool interior = wrow0 + offmin >= 0 && wrow0 + 1023 + offmax < cols;
     if (row >= rows) return;
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    for (int d = 0; d < ndiag; ++d) {
-        const int c = row + offsets[d];
-        const float4 v = ld_stream(data + (size_t)d * L4 + r4);
-        float x0, x1, x2, x3;
-        if (c >= 0 && c + 3 < cols) {
-            x0 = x[c];
-            x1 = x[c + 1];
-            x2 = x[c + 2];
-            x3 = x[c + 3];
-        } else {
-            x0 = xload(x, c, cols);
-            x1 = xload(x, c + 1, cols);
-            x2 = xload(x, c + 2, cols);
-            x3 = xload(x, c + 3, cols);
+    if (interior) {
+        int d = 0;
+        for (; d + 4 <= ndiag; d += 4) {
+            float4 v[4];
+            float xv[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                v[u] = ld_stream(data + (size_t)(d + u) * L4 + r4);
+                const float* xp = x + row + offsets[d + u];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xv[u][j] = xp[j];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a0 = fmaf(v[u].x, xv[u][0], a0);
+                a1 = fmaf(v[u].y, xv[u][1], a1);
+                a2 = fmaf(v[u].z, xv[u][2], a2);
+                a3 = fmaf(v[u].w, xv[u][3], a3);
+            }
         }
-        a0 = fmaf(v.x, x0, a0);
-        a1 = fmaf(v.y, x1, a1);
-        a2 = fmaf(v.z, x2, a2);
-        a3 = fmaf(v.w, x3, a3);
+        for (; d < ndiag; ++d) {
+            const float4 v = ld_stream(data + (size_t)d * L4 + r4);
+            const float* xp = x + row + offsets[d];
+            a0 = fmaf(v.x, xp[0], a0);
+            a1 = fmaf(v.y, xp[1], a1);
+            a2 = fmaf(v.z, xp[2], a2);
+            a3 = fmaf(v.w, xp[3], a3);
+        }
+    } else {
+        for (int d = 0; d < ndiag; ++d) {
+            const int c = row + offsets[d];
+            const float4 v = ld_stream(data + (size_t)d * L4 + r4);
+            a0 = fmaf(v.x, xload(x, c, cols), a0);
+            a1 = fmaf(v.y, xload(x, c + 1, cols), a1);
+            a2 = fmaf(v.z, xload(x, c + 2, cols), a2);
+            a3 = fmaf(v.w, xload(x, c + 3, cols), a3);
+        }
     }
     if (row + 3 < rows) {
         float4 yo = *reinterpret_cast<const float4*>(y + row);
@@ -141,7 +168,8 @@ int launch_dia(const DevDia& m, int method, const float* x, float* y, hipStream_
     if (method == 1 && (m.L % 4) == 0) {
         int r4 = (m.rows + 3) / 4;
         hipLaunchKernelGGL(dev::dia_row4_kernel, dim3((r4 + 255) / 256), dim3(256), 0, st, m.offsets,
-                           reinterpret_cast<const float4*>(m.data), m.L / 4, m.ndiag, x, y, m.rows, m.cols);
+                           reinterpret_cast<const float4*>(m.data), m.L / 4, m.ndiag, m.offmin, m.offmax, x, y,
+                           m.rows, m.cols);
     } else {
         hipLaunchKernelGGL(dev::dia_row_kernel, dim3((m.rows + 255) / 256), dim3(256), 0, st, m.offsets, m.data, m.L,
                            m.ndiag, x, y, m.rows, m.cols);

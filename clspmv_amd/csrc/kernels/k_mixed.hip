@@ -3,126 +3,256 @@
 //   kCodecBf16x2    v = hi + lo, two BF16 halves of one 32-bit word
 //                   (hi = bf16(v), lo = bf16(v - hi)): same bytes as FP32,
 //                   about 16 significant bits;
-//   kCodecFp16Row   v = s_row * h, h FP16, one FP32 scale per row (per block
-//                   row for BSR): 2 bytes per value;
+//   kCodecFp16Row   v = s_row * h, h FP16, one power-of-two FP32 scale per
+//                   row (per block row for BSR), applied to the row sums:
+//                   2 bytes per value;
 //   kCodecFp8Slice  v = s * q, q FP8 E4M3 (OCP: bias 7, 3 mantissa bits,
-//                   largest finite 448), one FP32 scale per 64-row slice of
-//                   each diagonal (DIA) or of each block element (BSR):
-//                   1 byte per value.
-// DIA: one row per lane (a wave's value loads per diagonal stay contiguous:
-// 256, 128 or 64 bytes).  BSR: the sliced layout of bsr_sliced_kernel
-// with the encoded values and, optionally, block columns stored as int16
-// deltas from the previous block of the same block row (first one from a
-// per-block-row base).
-#include <hip/hip_fp16.h>
-
+//                   largest finite 448), one FP32 scale per slice: 256 rows
+//                   of a diagonal (DIA) or 64 block rows of a block element
+//                   (BSR): 1 byte per value.
+// DIA: 1, 2 or 4 consecutive rows per lane, so that every lane loads one
+// 32-bit word of values per diagonal.  BSR: the sliced layout of
+// bsr_sliced_kernel with the encoded values and, optionally, block columns
+// stored as int16 deltas from the previous block of the same block row
+// (first one from a per-block-row base).
+#include "kcodec.hpp"
 #include "kcommon.hpp"
 #include "launch.hpp"
 
 namespace clspmv {
 namespace dev {
 
-__device__ __forceinline__ float bf16x2_value(uint32_t u) {
-    return __uint_as_float(u & 0xffff0000u) + __uint_as_float(u << 16);
-}
-__device__ __forceinline__ float f16_value(uint16_t h) { return __half2float(__ushort_as_half(h)); }
-// E4M3 decode from its definition: normal (e >= 1): 2^(e-7) * (1 + m/8),
-// subnormal (e = 0): m * 2^-9; the caller's scale absorbs the magnitude.
-__device__ __forceinline__ float e4m3_value(uint32_t q) {
-    const uint32_t e = (q >> 3) & 15u, m = q & 7u;
-    const float mag = e ? __uint_as_float(((e + 120u) << 23) | (m << 20)) : (float)m * (1.0f / 512.0f);
-    return (q & 0x80u) ? -mag : mag;
+// Loads of the 1- and 2-byte streams (FP16 / FP8 values, int16 columns):
+// plain loads -- with the nontemporal hint these streams ran 1.3-1.4x
+// slower on MI355X (c1a DIA FP16 35.9 vs 26.9 us, c1d BSR FP16 111 vs 81).
+constexpr bool kNarrowNontemporal = false;
+template <class T>
+__device__ __forceinline__ T ld_narrow(const T* p) {
+    if constexpr (kNarrowNontemporal) return ld_stream(p);
+    return *p;
 }
 
 __device__ __forceinline__ float xload_m(const float* __restrict__ x, int c, int cols) {
     return (c >= 0 && c < cols) ? x[c] : 0.f;
 }
 
-template <int CODEC>
-__device__ __forceinline__ float dia_value(const void* __restrict__ data, const float* __restrict__ scale, size_t i,
-                                          int d, int nsl, int row, float rs) {
-    if (CODEC == kCodecBf16x2) return bf16x2_value(ld_stream(reinterpret_cast<const uint32_t*>(data) + i));
-    if (CODEC == kCodecFp16Row) return rs * f16_value(reinterpret_cast<const uint16_t*>(data)[i]);
-    // the 64 rows of a wave share one slice: a wave-uniform (scalar) load
-    const int sl = __builtin_amdgcn_readfirstlane(row >> 6);
-    return scale[(size_t)d * nsl + sl] * e4m3_value(reinterpret_cast<const uint8_t*>(data)[i]);
+// R consecutive floats at p (p aligned to 4 R bytes) as one load / store.
+template <int R>
+__device__ __forceinline__ void load_vec(const float* p, float (&v)[R]) {
+    if constexpr (R == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(p);
+        v[0] = t.x, v[1] = t.y, v[2] = t.z, v[3] = t.w;
+    } else if constexpr (R == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(p);
+        v[0] = t.x, v[1] = t.y;
+    } else {
+        v[0] = p[0];
+    }
+}
+template <int R>
+__device__ __forceinline__ void store_vec(float* p, const float (&v)[R]) {
+    if constexpr (R == 4)
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    else if constexpr (R == 2)
+        *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    else
+        p[0] = v[0];
 }
 
-// One row per lane (as dia_row_kernel), diagonals four at a time so that
-// four value loads and four x loads are in flight per lane.
+// Rows per lane of the DIA kernel: every lane loads one 32-bit word of
+// values per diagonal (1 BF16 hi+lo word, 2 FP16 or 4 FP8 values).
+template <int CODEC>
+constexpr int dia_rows_per_lane() {
+    return CODEC == kCodecFp8Slice ? 4 : CODEC == kCodecFp16Row ? 2 : 1;
+}
+
+// The R values of rows row .. row+R-1 in one word, decoded; FP16 values are
+// returned unscaled (the row scale is applied to the row sums), FP8 values
+// times the slice scale s.
+template <int CODEC, int R>
+__device__ __forceinline__ void dia_decode(unsigned w, float s, float (&v)[R]) {
+    if constexpr (CODEC == kCodecBf16x2) {
+        v[0] = bf16x2_value(w);
+    } else if constexpr (CODEC == kCodecFp16Row) {
+        v[0] = f16_value((uint16_t)(w & 0xffffu));
+        v[1] = f16_value((uint16_t)(w >> 16));
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = s * e4m3_value((w >> (8 * j)) & 0xffu);
+    }
+}
+
+// R = dia_rows_per_lane consecutive rows per lane (a wave's value loads per
+// diagonal: 256 contiguous bytes for every codec) and R consecutive x
+// values per diagonal (one multi-dword load).  A workgroup covers 256 R
+// rows; when every column they can touch (its rows plus the smallest and
+// largest offset) lies inside x -- all but the first and last workgroups of
+// a banded matrix -- it takes the branch-free interior loop, chosen once per
+// workgroup.  With R = 4 a wave's 256 rows are one FP8 slice (a scalar
+// scale load per diagonal).
 template <int CODEC>
 __global__ __launch_bounds__(256) void dia_mixed_kernel(const int* __restrict__ offsets, const void* __restrict__ data,
                                                         const float* __restrict__ scale, int L, int ndiag, int nsl,
-                                                        const float* __restrict__ x, float* __restrict__ y, int rows,
-                                                        int cols) {
+                                                        int offmin, int offmax, const float* __restrict__ x,
+                                                        float* __restrict__ y, int rows, int cols) {
+    constexpr int R = dia_rows_per_lane<CODEC>();
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int row = tile * 256 + threadIdx.x;
+    const int row = (tile * 256 + threadIdx.x) * R;
+    const long long wrow0 = (long long)tile * 256 * R;
+    const bool interior = wrow0 + offmin >= 0 && wrow0 + 256 * R - 1 + offmax < cols;
     if (row >= rows) return;
-    const float rs = CODEC == kCodecFp16Row ? scale[row] : 1.f;
-    float acc = y[row];
-    int d = 0;
-    for (; d + 4 <= ndiag; d += 4) {
-        float v[4], xv[4];
+    const unsigned* words = reinterpret_cast<const unsigned*>(data);
+    const int sl = __builtin_amdgcn_readfirstlane(row >> 8);
+    auto slice_scale = [&](int d) { return CODEC == kCodecFp8Slice ? scale[(size_t)d * nsl + sl] : 1.f; };
+    auto word = [&](int d) {
+        const size_t i = ((size_t)d * L + row) / R;
+        if constexpr (CODEC == kCodecBf16x2) return ld_stream(words + i);
+        return words[i];
+    };
+    // y is read before the diagonals (its latency overlaps theirs), R values
+    // at once unless the lane holds the end of y
+    const bool full = row + R <= rows;
+    float yv[R];
+    if (full) {
+        load_vec<R>(y + row, yv);
+    } else {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            v[u] = dia_value<CODEC>(data, scale, (size_t)(d + u) * L + row, d + u, nsl, row, rs);
-            xv[u] = xload_m(x, row + offsets[d + u], cols);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc = fmaf(v[u], xv[u], acc);
+        for (int j = 0; j < R; ++j) yv[j] = row + j < rows ? y[row + j] : 0.f;
     }
-    for (; d < ndiag; ++d)
-        acc = fmaf(dia_value<CODEC>(data, scale, (size_t)d * L + row, d, nsl, row, rs), xload_m(x, row + offsets[d], cols),
-                   acc);
-    y[row] = acc;
+    float a[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) a[j] = 0.f;
+    if (interior) {
+        int d = 0;
+        for (; d + 4 <= ndiag; d += 4) {
+            unsigned w[4];
+            float xv[4][R];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                w[u] = word(d + u);
+                const float* xp = x + row + offsets[d + u];
+#pragma unroll
+                for (int j = 0; j < R; ++j) xv[u][j] = xp[j];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                float v[R];
+                dia_decode<CODEC, R>(w[u], slice_scale(d + u), v);
+#pragma unroll
+                for (int j = 0; j < R; ++j) a[j] = fmaf(v[j], xv[u][j], a[j]);
+            }
+        }
+        for (; d < ndiag; ++d) {
+            float v[R];
+            dia_decode<CODEC, R>(word(d), slice_scale(d), v);
+            const float* xp = x + row + offsets[d];
+#pragma unroll
+            for (int j = 0; j < R; ++j) a[j] = fmaf(v[j], xp[j], a[j]);
+        }
+    } else {
+        for (int d = 0; d < ndiag; ++d) {
+            float v[R];
+            dia_decode<CODEC, R>(word(d), slice_scale(d), v);
+            const int c = row + offsets[d];
+#pragma unroll
+            for (int j = 0; j < R; ++j) a[j] = fmaf(v[j], xload_m(x, c + j, cols), a[j]);
+        }
+    }
+    float rs[R];
+    if (CODEC == kCodecFp16Row) load_vec<R>(scale + row, rs);  // the scale array has L >= row + R entries
+#pragma unroll
+    for (int j = 0; j < R; ++j) yv[j] += CODEC == kCodecFp16Row ? a[j] * rs[j] : a[j];
+    if (full) {
+        store_vec<R>(y + row, yv);
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+            if (row + j < rows) y[row + j] = yv[j];
+    }
 }
 
+// Value e of the block in slot idx, decoded (FP8: times the scale of element
+// e in slice s; FP16: the row scale is applied by the caller).
+template <int CODEC>
+__device__ __forceinline__ float bsr_value(const void* __restrict__ data, const float* __restrict__ scale, size_t i,
+                                          int e, int nsl, int s) {
+    if (CODEC == kCodecBf16x2) return bf16x2_value(ld_stream(reinterpret_cast<const uint32_t*>(data) + i));
+    if (CODEC == kCodecFp16Row) return f16_value(ld_narrow(reinterpret_cast<const uint16_t*>(data) + i));
+    return scale[(size_t)e * nsl + s] * e4m3_value(ld_narrow(reinterpret_cast<const uint8_t*>(data) + i));
+}
+
+// As bsr_sliced_kernel (blocks G at a time, loads issued back to back, x
+// through load_xblock; cols >= BW), with encoded values and optionally
+// int16 column steps: the G steps are loaded together and summed in order.
 template <int BH, int BW, int CODEC, bool DELTA>
 __global__ __launch_bounds__(256) void bsr_mixed_kernel(const int* __restrict__ sp, const void* __restrict__ colv,
                                                         const int* __restrict__ base, const void* __restrict__ data,
                                                         const float* __restrict__ scale, int stride, int brows,
                                                         int nsl, const float* __restrict__ x, float* __restrict__ y,
                                                         int rows, int cols) {
+    constexpr int G = bsr_group(BH * BW);
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= brows) return;
     // a wave's 64 block rows are one slice: wave-uniform slice quantities
     const int s = __builtin_amdgcn_readfirstlane(r >> 6), t = r & 63;
     const int b = sp[s];
     const int width = (sp[s + 1] - b) >> 6;
-    const float rs = CODEC == kCodecFp16Row ? scale[r] : 1.f;
     float acc[BH];
 #pragma unroll
     for (int h = 0; h < BH; ++h) acc[h] = 0.f;
-    int c = DELTA ? base[r] : 0;
-#pragma unroll 3
-    for (int k = 0; k < width; ++k) {
-        const int idx = b + (k << 6) + t;
-        if (DELTA)
-            c += reinterpret_cast<const int16_t*>(colv)[idx];
-        else
-            c = reinterpret_cast<const int*>(colv)[idx];
-        float xv[BW];
+    int cprev = DELTA ? base[r] : 0;
+    auto load_col = [&](int idx) {
+        if (DELTA) return (int)ld_narrow(reinterpret_cast<const int16_t*>(colv) + idx);
+        return ld_stream(reinterpret_cast<const int*>(colv) + idx);
+    };
+    int k = 0;
+    for (; k + G <= width; k += G) {
+        int c[G];
 #pragma unroll
-        for (int w = 0; w < BW; ++w) xv[w] = (c + w < cols) ? x[c + w] : 0.f;
+        for (int u = 0; u < G; ++u) c[u] = load_col(b + ((k + u) << 6) + t);
+        if (DELTA) {
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                cprev += c[u];
+                c[u] = cprev;
+            }
+        }
+        float xv[G][BW], v[G][BH * BW];
+#pragma unroll
+        for (int u = 0; u < G; ++u) load_xblock<BW>(x, c[u], cols, xv[u]);
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+#pragma unroll
+            for (int e = 0; e < BH * BW; ++e)
+                v[u][e] = bsr_value<CODEC>(data, scale, (size_t)e * stride + b + ((k + u) << 6) + t, e, nsl, s);
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+#pragma unroll
+            for (int h = 0; h < BH; ++h)
+#pragma unroll
+                for (int w = 0; w < BW; ++w) acc[h] = fmaf(v[u][h * BW + w], xv[u][w], acc[h]);
+    }
+    for (; k < width; ++k) {
+        const int idx = b + (k << 6) + t;
+        int c = load_col(idx);
+        if (DELTA) {
+            cprev += c;
+            c = cprev;
+        }
+        float xv[BW];
+        load_xblock<BW>(x, c, cols, xv);
 #pragma unroll
         for (int h = 0; h < BH; ++h)
 #pragma unroll
-            for (int w = 0; w < BW; ++w) {
-                const size_t e = (size_t)(h * BW + w) * stride + idx;
-                float v;
-                if (CODEC == kCodecBf16x2)
-                    v = bf16x2_value(reinterpret_cast<const uint32_t*>(data)[e]);
-                else if (CODEC == kCodecFp16Row)
-                    v = rs * f16_value(reinterpret_cast<const uint16_t*>(data)[e]);
-                else
-                    v = scale[(size_t)(h * BW + w) * nsl + s] * e4m3_value(reinterpret_cast<const uint8_t*>(data)[e]);
-                acc[h] = fmaf(v, xv[w], acc[h]);
-            }
+            for (int w = 0; w < BW; ++w)
+                acc[h] = fmaf(bsr_value<CODEC>(data, scale, (size_t)(h * BW + w) * stride + idx, h * BW + w, nsl, s),
+                              xv[w], acc[h]);
     }
+    const float rs = CODEC == kCodecFp16Row ? scale[r] : 1.f;
 #pragma unroll
     for (int h = 0; h < BH; ++h) {
         const int row = r * BH + h;
-        if (row < rows) y[row] += acc[h];
+        if (row < rows) y[row] += CODEC == kCodecFp16Row ? rs * acc[h] : acc[h];
     }
 }
 
@@ -130,21 +260,17 @@ __global__ __launch_bounds__(256) void bsr_mixed_kernel(const int* __restrict__ 
 
 int launch_dia_mixed(const DevDiaMixed& m, const float* x, float* y, hipStream_t st) {
     if (m.rows == 0) return 0;
-    const int groups = (m.rows + 255) / 256;
+#define CLSPMV_DIA_MIXED(C)                                                                                     \
+    hipLaunchKernelGGL(dev::dia_mixed_kernel<C>, dim3((m.rows + 256 * dev::dia_rows_per_lane<C>() - 1) /      \
+                                                      (256 * dev::dia_rows_per_lane<C>())),                   \
+                       dim3(256), 0, st, m.offsets, m.data, m.scale, m.L, m.ndiag, m.nsl, m.offmin, m.offmax, x, \
+                       y, m.rows, m.cols)
     switch (m.codec) {
-        case kCodecBf16x2:
-            hipLaunchKernelGGL(dev::dia_mixed_kernel<kCodecBf16x2>, dim3(groups), dim3(256), 0, st, m.offsets, m.data,
-                               m.scale, m.L, m.ndiag, m.nsl, x, y, m.rows, m.cols);
-            break;
-        case kCodecFp16Row:
-            hipLaunchKernelGGL(dev::dia_mixed_kernel<kCodecFp16Row>, dim3(groups), dim3(256), 0, st, m.offsets,
-                               m.data, m.scale, m.L, m.ndiag, m.nsl, x, y, m.rows, m.cols);
-            break;
-        default:
-            hipLaunchKernelGGL(dev::dia_mixed_kernel<kCodecFp8Slice>, dim3(groups), dim3(256), 0, st, m.offsets,
-                               m.data, m.scale, m.L, m.ndiag, m.nsl, x, y, m.rows, m.cols);
-            break;
+        case kCodecBf16x2: CLSPMV_DIA_MIXED(kCodecBf16x2); break;
+        case kCodecFp16Row: CLSPMV_DIA_MIXED(kCodecFp16Row); break;
+        default: CLSPMV_DIA_MIXED(kCodecFp8Slice); break;
     }
+#undef CLSPMV_DIA_MIXED
     return (int)hipGetLastError();
 }
 

@@ -31,6 +31,13 @@
 // deterministic.  Every workgroup streams all of x it touches through LDS,
 // so the x traffic is (row blocks) x (slice bytes) from L2/MALL, in
 // full cache lines, against one L2 request per nonzero for a gather.
+//
+// Mixed-precision values (the CODEC template argument; FP32 products and
+// sums): BF16 hi + lo words (4 bytes per entry, as FP32), FP16 divided by a
+// power-of-two row scale that is applied once per row when the block's sums
+// leave LDS (2 bytes), or FP8 E4M3 times one FP32 scale per 128-entry chunk
+// (1 byte).
+#include "kcodec.hpp"
 #include "kcommon.hpp"
 #include "launch.hpp"
 
@@ -135,12 +142,34 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
     if (lane == 63) xt_emit(tail_row, cval, first_row, y_l, cr, cv, wave);
 }
 
+// Per-lane value storage of one chunk (two entries) for each codec.
+template <int CODEC> struct XtVal { typedef f32x2 T; };
+template <> struct XtVal<kCodecBf16x2> { typedef u32x2 T; };
+template <> struct XtVal<kCodecFp16Row> { typedef unsigned T; };        // entry 0 in the low half
+template <> struct XtVal<kCodecFp8Slice> { typedef unsigned short T; };  // entry 0 in the low byte
+
+template <int CODEC>
+__device__ __forceinline__ f32x2 xt_decode(typename XtVal<CODEC>::T v, float s) {
+    if constexpr (CODEC == kCodecBf16x2) {
+        return f32x2{bf16x2_value(v.x), bf16x2_value(v.y)};
+    } else if constexpr (CODEC == kCodecFp16Row) {
+        return f32x2{f16_value((uint16_t)(v & 0xffffu)), f16_value((uint16_t)(v >> 16))};
+    } else if constexpr (CODEC == kCodecFp8Slice) {
+        return f32x2{s * e4m3_value(v & 0xffu), s * e4m3_value((unsigned)v >> 8)};
+    } else {
+        (void)s;
+        return v;
+    }
+}
+
 // Registers holding one tile ahead of use: its x slice (unless the tile
 // continues the previous tile's slice) and this wavefront's chunks.
+template <int CODEC>
 struct XtSet {
     f32x4u x[kXtCols / (4 * kXtThreads)];
     u32x2 e[kXtRounds];
-    f32x2 v[kXtRounds];
+    typename XtVal<CODEC>::T v[kXtRounds];
+    float s[kXtRounds];  // FP8 chunk scales
 };
 
 struct XtArgs {
@@ -149,7 +178,8 @@ struct XtArgs {
     const int* tile_slice;  // (slice << 1) | continues-previous-slice
     const int* tile_chunk;
     const u32x2* idx2;
-    const f32x2* val2;
+    const void* val;      // chunk values, XtVal<CODEC>::T per lane
+    const float* vscale;  // FP16: row scales; FP8: chunk scales
     int cols, last_chunk;
     const float* x;
     float* y;
@@ -158,18 +188,23 @@ struct XtArgs {
 // Loads are issued unconditionally (indices clamped to valid ones) and each
 // register set is refilled in place right after its last use, so that the
 // wait before each use covers only the loads issued before it.
-__device__ __forceinline__ void xt_fetch_x(XtSet& S, const XtArgs& a, int t, int tid) {
+template <int CODEC>
+__device__ __forceinline__ void xt_fetch_x(XtSet<CODEC>& S, const XtArgs& a, int t, int tid) {
     const int base = (a.tile_slice[t] >> 1) * kXtCols;
 #pragma unroll
     for (int q = 0; q < kXtCols / (4 * kXtThreads); ++q) S.x[q] = load_x4(a.x, base + (q * kXtThreads + tid) * 4, a.cols);
 }
-__device__ __forceinline__ void xt_fetch_e(XtSet& S, const XtArgs& a, int t, int wave, int lane) {
+template <int CODEC>
+__device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int t, int wave, int lane) {
+    typedef typename XtVal<CODEC>::T V;
     const int c0 = a.tile_chunk[t];
 #pragma unroll
     for (int k = 0; k < kXtRounds; ++k) {
-        const int c = min(c0 + wave + k * kXtWaves, a.last_chunk);
+        // wave-uniform chunk index (the FP8 scale becomes a scalar load)
+        const int c = __builtin_amdgcn_readfirstlane(min(c0 + wave + k * kXtWaves, a.last_chunk));
         S.e[k] = __builtin_nontemporal_load(a.idx2 + (size_t)c * 64 + lane);
-        S.v[k] = __builtin_nontemporal_load(a.val2 + (size_t)c * 64 + lane);
+        S.v[k] = __builtin_nontemporal_load(reinterpret_cast<const V*>(a.val) + (size_t)c * 64 + lane);
+        if constexpr (CODEC == kCodecFp8Slice) S.s[k] = a.vscale[c];
     }
 }
 
@@ -182,7 +217,8 @@ struct XtState {
 // reduce its chunks.  Steps past the block's last tile (t >= t1) only keep
 // the barrier sequence and the carry bookkeeping, so that the unrolled loop
 // issues every load on every path.
-__device__ __forceinline__ void xt_step(XtSet& S, const XtArgs& a, int t, int t1, float* y_l, float4* x_l4,
+template <int CODEC>
+__device__ __forceinline__ void xt_step(XtSet<CODEC>& S, const XtArgs& a, int t, int t1, float* y_l, float4* x_l4,
                                         int (*c_row)[kXtWaves], float (*c_val)[kXtWaves], XtState& st, int tid,
                                         int wave, int lane) {
     __syncthreads();  // previous tile done with x_l; its carry slots are written
@@ -219,7 +255,8 @@ __device__ __forceinline__ void xt_step(XtSet& S, const XtArgs& a, int t, int t1
                 if (tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
             }
             if (cb + wave < c1)
-                xt_chunk(S.e[k], S.v[k], x_l, y_l, c_row[st.par], c_val[st.par], wave, lane);
+                xt_chunk(S.e[k], xt_decode<CODEC>(S.v[k], S.s[k]), x_l, y_l, c_row[st.par], c_val[st.par], wave,
+                         lane);
             else if (lane == 0)
                 c_row[st.par][wave] = -1;
             st.par ^= 1;
@@ -229,14 +266,15 @@ __device__ __forceinline__ void xt_step(XtSet& S, const XtArgs& a, int t, int t1
     xt_fetch_e(S, a, tn, wave, lane);
 }
 
+template <int CODEC>
 __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     const int* __restrict__ blk_row, const int* __restrict__ blk_tile, const int* __restrict__ tile_slice,
-    const int* __restrict__ tile_chunk, const u32x2* __restrict__ idx2, const f32x2* __restrict__ val2, int cols,
-    int last_chunk, const float* __restrict__ x, float* __restrict__ y, float* __restrict__ partial, int groups,
-    int rows) {
+    const int* __restrict__ tile_chunk, const u32x2* __restrict__ idx2, const void* __restrict__ val,
+    const float* __restrict__ vscale, int cols, int last_chunk, const float* __restrict__ x, float* __restrict__ y,
+    float* __restrict__ partial, int groups, int rows) {
     // the metadata is read-only and unaliased, so its wave-uniform reads
     // become scalar loads that do not wait on the vector-load queue
-    const XtArgs a{blk_row, blk_tile, tile_slice, tile_chunk, idx2, val2, cols, last_chunk, x, y};
+    const XtArgs a{blk_row, blk_tile, tile_slice, tile_chunk, idx2, val, vscale, cols, last_chunk, x, y};
     __shared__ float y_l[kXtRows];
     __shared__ float4 x_l4[kXtCols / 4];
     __shared__ int c_row[2][kXtWaves];
@@ -255,7 +293,7 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     for (int r = tid; r < nr; r += kXtThreads) y_l[r] = 0.f;
 
     static_assert(kXtDepth == 3, "the tile loop below is unrolled for three register sets");
-    XtSet A, B, C;
+    XtSet<CODEC> A, B, C;
     xt_fetch_x(A, a, t0, tid);
     xt_fetch_e(A, a, t0, wave, lane);
     xt_fetch_x(B, a, min(t0 + 1, t1 - 1), tid);
@@ -271,6 +309,8 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     __syncthreads();
     if (st.pending && tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
     __syncthreads();
+    if constexpr (CODEC == kCodecFp16Row)
+        for (int r = tid; r < nr; r += kXtThreads) y_l[r] *= a.vscale[r0 + r];
     if (out) {
         for (int r = tid; r < nr; r += kXtThreads) out[r] = y_l[r];
     } else {
@@ -292,10 +332,17 @@ __global__ __launch_bounds__(256) void xtile_reduce_kernel(const float* __restri
 
 int launch_xtile(const DevXtile& m, const float* x, float* y, hipStream_t st) {
     if (m.nblocks == 0) return 0;
-    hipLaunchKernelGGL(dev::xtile_kernel, dim3(m.nblocks), dim3(kXtThreads), 0, st, m.blk_row, m.blk_tile,
-                       m.tile_slice, m.tile_chunk, reinterpret_cast<const dev::u32x2*>(m.idx),
-                       reinterpret_cast<const dev::f32x2*>(m.val), m.cols, m.nchunks - 1, x, y, m.partial, m.groups,
-                       m.rows);
+#define CLSPMV_XTILE(C)                                                                                          \
+    hipLaunchKernelGGL(dev::xtile_kernel<C>, dim3(m.nblocks), dim3(kXtThreads), 0, st, m.blk_row, m.blk_tile, \
+                       m.tile_slice, m.tile_chunk, reinterpret_cast<const dev::u32x2*>(m.idx), m.val, m.vscale,   \
+                       m.cols, m.nchunks - 1, x, y, m.partial, m.groups, m.rows)
+    switch (m.codec) {
+        case kCodecBf16x2: CLSPMV_XTILE(kCodecBf16x2); break;
+        case kCodecFp16Row: CLSPMV_XTILE(kCodecFp16Row); break;
+        case kCodecFp8Slice: CLSPMV_XTILE(kCodecFp8Slice); break;
+        default: CLSPMV_XTILE(0); break;
+    }
+#undef CLSPMV_XTILE
     int e = (int)hipGetLastError();
     if (e || m.groups == 1) return e;
     hipLaunchKernelGGL(dev::xtile_reduce_kernel, dim3((m.rows + 255) / 256), dim3(256), 0, st, m.partial, m.groups,

@@ -25,6 +25,34 @@ __device__ __forceinline__ float4 ld_stream(const float4* p) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// Blocks per step of the sliced BSR kernels (FP32 and mixed precision):
+// the column, x and value loads of that many blocks are issued together.
+constexpr int kBsrGroupSmall = 4;  // blocks of <= 16 values
+__host__ __device__ constexpr int bsr_group(int values_per_block) {
+    return values_per_block <= 16 ? kBsrGroupSmall : 2;
+}
+
+// x[c .. c+BW-1] for a block starting at column c, zero at and past `cols`
+// (requires cols >= BW): BW consecutive floats are read from
+// min(c, cols - BW) -- one multi-dword load, no branch -- and shifted into
+// place; the shift is nonzero only for a block overlapping the end of x.
+template <int BW>
+__device__ __forceinline__ void load_xblock(const float* __restrict__ x, int c, int cols, float (&xv)[BW]) {
+    const int cb = min(c, cols - BW);
+    float v[BW];
+#pragma unroll
+    for (int w = 0; w < BW; ++w) v[w] = x[cb + w];
+    const int d = c - cb;
+#pragma unroll
+    for (int w = 0; w < BW; ++w) {
+        float o = 0.f;
+#pragma unroll
+        for (int j = 0; j < BW; ++j)
+            if (w + j < BW) o = (d == j) ? v[w + j] : o;
+        xv[w] = o;
+    }
+}
+
 // Sum over all 64 lanes; every lane receives the total.
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll

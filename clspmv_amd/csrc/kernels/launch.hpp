@@ -65,6 +65,7 @@ struct DevDia {
     const int* offsets;
     const float* data;
     int L, ndiag, rows, cols;
+    int offmin, offmax;  // smallest and largest diagonal offset
 };
 struct DevBdia {
     const int* bptr;
@@ -97,6 +98,11 @@ struct DevBsr {
 };
 constexpr int kBsrSlice = 64;  // block rows per slice of the sliced BSR layout
 
+// Mixed-precision parts (k_mixed.hip, k_xtile.hip): value codecs, as CLSPMV_VAL_* in the ABI.
+constexpr int kCodecBf16x2 = 1;
+constexpr int kCodecFp16Row = 2;
+constexpr int kCodecFp8Slice = 3;
+
 // LDS x-tiled layout (k_xtile.hip): row blocks of <= kXtRows rows, column
 // slices of kXtCols columns, entries packed (row_in_block << 16 |
 // col_in_slice) in chunks of kXtChunk; padding entries carry row kXtPadRow.
@@ -116,22 +122,21 @@ struct DevXtile {
     const int* tile_slice;  // (column slice << 1) | 1 if the tile continues the previous tile's slice
     const int* tile_chunk;  // ntiles + 1 chunk ranges, at most kXtTileChunks per tile
     const unsigned* idx;
-    const float* val;
+    const void* val;        // FP32 values, or encoded with `codec` (k_xtile.hip)
+    const float* vscale;    // FP16: rows row scales; FP8: nchunks chunk scales
     float* partial;         // groups * rows partial sums when groups > 1
     int nblocks;            // workgroups = row blocks * groups
     int groups;             // column groups (1, 2, 4 or 8); workgroup b works on group b % groups
     int ntiles, nchunks, cols, rows;
+    int codec;              // 0 (FP32) or a kCodec* value
 };
 
-// Mixed-precision parts (k_mixed.hip): value codecs, as CLSPMV_VAL_* in the ABI.
-constexpr int kCodecBf16x2 = 1;
-constexpr int kCodecFp16Row = 2;
-constexpr int kCodecFp8Slice = 3;
 struct DevDiaMixed {
     const int* offsets;
     const void* data;     // ndiag * L encoded values
-    const float* scale;   // FP16: L row scales; FP8: ndiag * nsl slice scales
+    const float* scale;   // FP16: L row scales; FP8: ndiag * nsl scales of 256-row slices
     int L, ndiag, nsl, rows, cols, codec;
+    int offmin, offmax;   // smallest and largest diagonal offset
 };
 struct DevBsrMixed {
     const int* sp;        // slice pointer of the sliced layout

@@ -87,6 +87,7 @@ struct clspmv_part_s {
     DevXtile xt{};
     DevDiaMixed diam{};
     DevBsrMixed bsrm{};
+    bool mixed = false;  // a mixed-precision part (diam, bsrm, or xt with a codec)
 
     void* add(clspmv_ctx_t c, const void* host, size_t bytes, int* rc) {
         bufs.emplace_back();
@@ -148,9 +149,31 @@ int make_part(clspmv_ctx_t ctx, clspmv_part_t* out, F build) {
     return CLSPMV_OK;
 }
 
+constexpr int kMethodMixed = 16;  // part method = 16 + codec (+ 4 with int16 delta columns)
+
+bool codec_ok(int codec) { return codec == kCodecBf16x2 || codec == kCodecFp16Row || codec == kCodecFp8Slice; }
+
+// BF16 hi + lo split of v: (bf16(v) << 16) | bf16(v - bf16(v)).
+uint32_t bf16x2_word(float v) {
+    const uint16_t hi = f32_to_bf16(v);
+    const uint16_t lo = f32_to_bf16(v - bf16_to_f32(hi));
+    return ((uint32_t)hi << 16) | lo;
+}
+
+// FP8 scale: the largest magnitude maps to the largest finite E4M3 (448).
+float fp8_scale(float amax) { return amax > 0.f ? amax / 448.0f : 1.0f; }
+// FP16 row scale: a power of two (so v / s is exact) that puts the row's
+// largest magnitude in [2^14, 2^15), well inside FP16's normal range.
+float fp16_scale(float amax) { return amax > 0.f ? std::ldexp(1.0f, std::ilogb(amax) - 14) : 1.0f; }
+
+const char* codec_name(int codec) {
+    return codec == kCodecBf16x2 ? "bf16x2" : codec == kCodecFp16Row ? "fp16row" : "fp8slice";
+}
+
 int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
                      int cols);
-int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows, int cols);
+int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows, int cols,
+                int codec = 0);
 
 // Row-list view (CSR arrays) of a stored format, for the layouts that are
 // rebuilt from rows (column-sliced, x-tiled).  Every stored slot is kept,
@@ -194,8 +217,10 @@ int finish_sliced(clspmv_part_t p, int fmt, int rows, int cols, long long nnz, c
 
 int finish_xtile(clspmv_part_t p, int fmt, int rows, int cols, long long nnz) {
     char name[48];
-    std::snprintf(name, sizeof(name), "xtile_B%d_G%d", p->xt.nblocks / p->xt.groups, p->xt.groups);
-    fill_info(p, fmt, kMethodXtile, rows, cols, nnz, name, p->xt.groups > 1 ? 2 : 1);
+    const int c = p->xt.codec;
+    std::snprintf(name, sizeof(name), "xtile%s%s_B%d_G%d", c ? "_" : "", c ? codec_name(c) : "",
+                  p->xt.nblocks / p->xt.groups, p->xt.groups);
+    fill_info(p, fmt, c ? kMethodMixed + c : kMethodXtile, rows, cols, nnz, name, p->xt.groups > 1 ? 2 : 1);
     // the partial-sum buffer is scratch, not format storage
     p->info.stored_bytes -= p->carry_bytes;
     p->info.alg_bytes -= p->carry_bytes;
@@ -563,8 +588,10 @@ int g_xt_max_groups = 1;
 // balanced by entry count (at most kXtRows rows each); within a block the
 // entries are bucketed by column slice in row order, so each tile is sorted
 // by (row, col) when the rows are.
+// `codec` (0 or a kCodec* value) encodes the values: FP16 divided by a
+// power-of-two scale per row, FP8 with one scale per 128-entry chunk.
 int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
-                int cols) {
+                int cols, int codec) {
     const long long nnz = rows ? rp[rows] : 0;
     if (cols < 4) {
         set_error("the x-tiled layout needs at least 4 columns");
@@ -623,6 +650,16 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
     const size_t total = (size_t)tile_chunk.back() * kXtChunk;
     std::vector<unsigned> idx(std::max<size_t>(total, 1), kXtPadRow << 16);
     std::vector<float> val(std::max<size_t>(total, 1), 0.f);
+    std::vector<float> rscale;
+    if (codec == kCodecFp16Row) {
+        rscale.assign((size_t)std::max(rows, 1), 1.0f);
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+        for (int r = 0; r < rows; ++r) {
+            float a = 0.f;
+            for (long long k = rp[r]; k < rp[r + 1]; ++k) a = std::max(a, std::fabs(cv[k]));
+            rscale[(size_t)r] = fp16_scale(a);
+        }
+    }
 #pragma omp parallel num_threads(omp_threads())
     {
         std::vector<long long> pos((size_t)ns, -1);
@@ -637,7 +674,7 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
                     const int s = ci[k] / kXtCols;
                     const long long q = pos[(size_t)s]++;
                     idx[(size_t)q] = ((unsigned)(r - brow[b]) << 16) | (unsigned)(ci[k] - s * kXtCols);
-                    val[(size_t)q] = cv[k];
+                    val[(size_t)q] = rscale.empty() ? cv[k] : cv[k] / rscale[(size_t)r];
                 }
         }
     }
@@ -653,8 +690,41 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
     if (rc) return rc;
     d.idx = (const unsigned*)p->add(ctx, idx.data(), idx.size() * 4, &rc);
     if (rc) return rc;
-    d.val = (const float*)p->add(ctx, val.data(), val.size() * 4, &rc);
+    std::vector<float> vscale;
+    if (codec == kCodecBf16x2) {
+        std::vector<uint32_t> w(val.size());
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+        for (long long i = 0; i < (long long)val.size(); ++i) w[(size_t)i] = bf16x2_word(val[(size_t)i]);
+        d.val = p->add(ctx, w.data(), w.size() * 4, &rc);
+    } else if (codec == kCodecFp16Row) {
+        std::vector<uint16_t> h(val.size());
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+        for (long long i = 0; i < (long long)val.size(); ++i) h[(size_t)i] = f32_to_f16(val[(size_t)i]);
+        d.val = p->add(ctx, h.data(), std::max<size_t>(h.size(), 2) * 2, &rc);
+        vscale.swap(rscale);
+    } else if (codec == kCodecFp8Slice) {
+        const long long nc = std::max(1, tile_chunk.back());
+        std::vector<uint8_t> q((size_t)nc * kXtChunk, 0);
+        vscale.assign((size_t)nc, 1.0f);
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+        for (long long c = 0; c < nc; ++c) {
+            const float* v = val.data() + (size_t)c * kXtChunk;
+            float a = 0.f;
+            for (int i = 0; i < kXtChunk; ++i) a = std::max(a, std::fabs(v[i]));
+            const float sc = fp8_scale(a);
+            vscale[(size_t)c] = sc;
+            for (int i = 0; i < kXtChunk; ++i) q[(size_t)c * kXtChunk + (size_t)i] = f32_to_e4m3(v[i] / sc);
+        }
+        d.val = p->add(ctx, q.data(), q.size(), &rc);
+    } else {
+        d.val = p->add(ctx, val.data(), val.size() * 4, &rc);
+    }
     if (rc) return rc;
+    if (!vscale.empty()) {
+        d.vscale = (const float*)p->add(ctx, vscale.data(), vscale.size() * 4, &rc);
+        if (rc) return rc;
+    }
+    d.codec = codec;
     if (G > 1) {
         d.partial = (float*)p->scratch((size_t)G * rows * 4, &rc);
         if (rc) return rc;
@@ -847,6 +917,8 @@ int clspmv_part_dia(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int method, cl
         p->dia.ndiag = D;
         p->dia.rows = rows;
         p->dia.cols = cols;
+        p->dia.offmin = D ? *std::min_element(m->dia_offsets, m->dia_offsets + D) : 0;
+        p->dia.offmax = D ? *std::max_element(m->dia_offsets, m->dia_offsets + D) : 0;
         int meth = (method == 1 || method == 3) ? 1 : 0;
         if (L % 4) meth = 0;
         fill_info(p, CLSPMV_FMT_DIA, meth, rows, cols, m->matinfo.nnz, meth ? "dia_row4" : "dia_row", 1);
@@ -1010,6 +1082,7 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
         }
         int rc = 0;
         const unsigned bs = bw * bh;
+        if (method == 0 && cols < (int)bw) method = 1;  // the sliced kernel reads bw columns at a time
         if (method == 0) {
             // Sliced device layout (see bsr_sliced_kernel): built here from
             // the block-CSR host arrays.
@@ -1073,40 +1146,18 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
 
 // ---------------------------------------------------------------- mixed precision
 
-namespace {
-
-constexpr int kMethodMixed = 16;  // part method = 16 + codec (+ 4 with int16 delta columns)
-
-bool codec_ok(int codec) { return codec == kCodecBf16x2 || codec == kCodecFp16Row || codec == kCodecFp8Slice; }
-
-// Encodes n values v[i * step] (i < n) with `codec` into out (bytes per
-// value: 4, 2, 1); FP16 divides by `s` first, FP8 by its own scale.
-uint32_t bf16x2_word(float v) {
-    const uint16_t hi = f32_to_bf16(v);
-    const uint16_t lo = f32_to_bf16(v - bf16_to_f32(hi));
-    return ((uint32_t)hi << 16) | lo;
-}
-
-float fp8_scale(float amax) { return amax > 0.f ? amax / 448.0f : 1.0f; }
-
-const char* codec_name(int codec) {
-    return codec == kCodecBf16x2 ? "bf16x2" : codec == kCodecFp16Row ? "fp16row" : "fp8slice";
-}
-
-}  // namespace
-
 extern "C" {
 
 int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int codec, clspmv_part_t* out) {
     return make_part(ctx, out, [&](clspmv_part_t p) {
         if (!codec_ok(codec)) return CLSPMV_E_INVALID;
         const int rows = m->matinfo.height, cols = m->matinfo.width, L = m->dia_length_aligned, nd = m->dia_num;
-        if (rows < 0 || L < rows || nd < 0 || (L % 64) != 0) {
-            set_error("mixed DIA needs dia_length_aligned a multiple of 64 (coo2dia alignment 64)");
+        if (rows < 0 || L < rows || nd < 0 || (L % 256) != 0) {
+            set_error("mixed DIA needs dia_length_aligned a multiple of 256 (coo2dia alignment 256)");
             return CLSPMV_E_INVALID;
         }
         const size_t n = (size_t)nd * L;
-        const int nsl = L / 64;
+        const int nsl = L / 256;  // FP8 slices of 256 rows
         std::vector<float> scale;
         std::vector<unsigned char> enc;
         if (codec == kCodecBf16x2) {
@@ -1119,7 +1170,7 @@ int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int code
             for (int r = 0; r < L; ++r) {
                 float a = 0.f;
                 for (int d = 0; d < nd; ++d) a = std::max(a, std::fabs(m->dia_data[(size_t)d * L + r]));
-                if (a > 0.f) scale[(size_t)r] = a;
+                scale[(size_t)r] = fp16_scale(a);
             }
             std::vector<uint16_t> h(n);
 #pragma omp parallel for schedule(static) num_threads(omp_threads())
@@ -1132,12 +1183,12 @@ int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int code
 #pragma omp parallel for schedule(static) num_threads(omp_threads())
             for (long long q = 0; q < (long long)nd * nsl; ++q) {
                 const size_t d = (size_t)q / nsl, s = (size_t)q % nsl;
-                const float* v = m->dia_data + d * L + s * 64;
+                const float* v = m->dia_data + d * L + s * 256;
                 float a = 0.f;
-                for (int i = 0; i < 64; ++i) a = std::max(a, std::fabs(v[i]));
+                for (int i = 0; i < 256; ++i) a = std::max(a, std::fabs(v[i]));
                 const float sc = fp8_scale(a);
                 scale[(size_t)q] = sc;
-                for (int i = 0; i < 64; ++i) enc[d * L + s * 64 + (size_t)i] = f32_to_e4m3(v[i] / sc);
+                for (int i = 0; i < 256; ++i) enc[d * L + s * 256 + (size_t)i] = f32_to_e4m3(v[i] / sc);
             }
         }
         int rc = 0;
@@ -1156,7 +1207,10 @@ int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int code
         d.rows = rows;
         d.cols = cols;
         d.codec = codec;
+        d.offmin = nd ? *std::min_element(m->dia_offsets, m->dia_offsets + nd) : 0;
+        d.offmax = nd ? *std::max_element(m->dia_offsets, m->dia_offsets + nd) : 0;
         p->diam = d;
+        p->mixed = true;
         char name[40];
         std::snprintf(name, sizeof(name), "dia_mixed_%s", codec_name(codec));
         fill_info(p, CLSPMV_FMT_DIA, kMethodMixed + codec, rows, cols, m->matinfo.nnz, name, 1);
@@ -1171,6 +1225,10 @@ int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int code
         const unsigned bw = m->bsr_bwidth, bh = m->bsr_bheight;
         if (bw != bh || bw < 2 || bw > 4) return CLSPMV_E_UNSUPPORTED;
         const int rows = m->matinfo.height, cols = m->matinfo.width;
+        if (cols < (int)bw) {
+            set_error("mixed BSR needs at least bw columns");
+            return CLSPMV_E_UNSUPPORTED;
+        }
         const int brows = m->bsr_row_num, nb = m->bsr_block_num, stride = m->bsr_block_stride;
         if (brows != (rows + (int)bh - 1) / (int)bh || stride < nb || m->bsr_row_ptr[0] != 0 ||
             m->bsr_row_ptr[brows] != nb)
@@ -1244,7 +1302,7 @@ int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int code
                 for (int k = 0; k < width; ++k)
                     for (unsigned q = 0; q < bs; ++q)
                         amax = std::max(amax, std::fabs(sdat[(size_t)q * sstride + (size_t)sp[s] + (size_t)k * S + t]));
-                if (amax > 0.f) scale[(size_t)r] = amax;
+                scale[(size_t)r] = fp16_scale(amax);
             }
             std::vector<uint16_t> h(nv, 0);
             for (int r = 0; r < brows; ++r) {
@@ -1297,10 +1355,22 @@ int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int code
         d.codec = codec;
         d.delta = delta ? 1 : 0;
         p->bsrm = d;
+        p->mixed = true;
         char name[48];
         std::snprintf(name, sizeof(name), "bsr%u%u_mixed_%s%s", bh, bw, codec_name(codec), delta ? "_d16" : "");
         fill_info(p, CLSPMV_FMT_BSR, kMethodMixed + codec + (delta ? 4 : 0), rows, cols, m->matinfo.nnz, name, 1);
         return CLSPMV_OK;
+    });
+}
+
+int clspmv_part_csr_mixed(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int codec, clspmv_part_t* out) {
+    return make_part(ctx, out, [&](clspmv_part_t p) {
+        if (!codec_ok(codec)) return CLSPMV_E_INVALID;
+        const int rows = m->matinfo.height, cols = m->matinfo.width;
+        const int rc = build_xtile(ctx, p, m->csr_row_ptr, m->csr_col_id, m->csr_data, rows, cols, codec);
+        if (rc) return rc;
+        p->mixed = true;
+        return finish_xtile(p, CLSPMV_FMT_CSR_VECTOR, rows, cols, m->matinfo.nnz);
     });
 }
 
@@ -1331,7 +1401,8 @@ int clspmv_set_option(const char* name, long long value) {
 }
 
 int clspmv_part_launch(clspmv_part_t p, const float* x, float* y, hipStream_t st) {
-    if (p->info.method >= kMethodMixed) {
+    if (p->mixed) {
+        if (p->xt.codec) return launch_xtile(p->xt, x, y, st);
         if (p->info.format == CLSPMV_FMT_DIA) return launch_dia_mixed(p->diam, x, y, st);
         if (p->info.format == CLSPMV_FMT_BSR) return launch_bsr_mixed(p->bsrm, x, y, st);
         return (int)hipErrorInvalidValue;

@@ -443,8 +443,9 @@ class Part:
 
 
 class MixedPart(Part):
-    """A mixed-precision part (clspmv_part_dia_mixed / clspmv_part_bsr_mixed):
-    values stored with `codec` (L.VAL_*), FP32 accumulation."""
+    """A mixed-precision part (clspmv_part_dia_mixed / clspmv_part_bsr_mixed /
+    clspmv_part_csr_mixed): values stored with `codec` (L.VAL_*), FP32
+    accumulation.  CSR parts use the LDS x-tiled layout."""
 
     def __init__(self, ctx: Context, fmt: int, mat, codec: int, delta_cols: bool = True):
         self.ctx = ctx
@@ -454,8 +455,10 @@ class MixedPart(Part):
         elif fmt == L.FMT_BSR:
             check(lib().clspmv_part_bsr_mixed(ctx.h, C.byref(mat.s), codec, int(delta_cols), C.byref(self.h)),
                   "part_bsr_mixed")
+        elif fmt in (L.FMT_CSR_VECTOR, L.FMT_CSR_SCALAR):
+            check(lib().clspmv_part_csr_mixed(ctx.h, C.byref(mat.s), codec, C.byref(self.h)), "part_csr_mixed")
         else:
-            raise ValueError("mixed precision is provided for DIA and BSR parts")
+            raise ValueError("mixed precision is provided for DIA, BSR and CSR parts")
         inf = L.PartInfo()
         check(lib().clspmv_part_get_info(self.h, C.byref(inf)), "part_get_info")
         self.info = {k: getattr(inf, k) for k, _ in L.PartInfo._fields_}

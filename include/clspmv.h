@@ -329,19 +329,24 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
 /* Mixed-precision parts (no counterpart in the reference; for callers that
  * trade value precision for bytes per nonzero): values stored as
  *   CLSPMV_VAL_BF16X2    hi + lo, two BF16 halves (4 B, ~16 significant bits)
- *   CLSPMV_VAL_FP16_ROW  FP16 times one FP32 scale per row (block row) (2 B)
- *   CLSPMV_VAL_FP8_SLICE FP8 E4M3 (OCP) times one FP32 scale per 64-row slice
- *                        of each diagonal / block element (1 B)
- * accumulated in FP32.  DIA: dia_length_aligned must be a multiple of 64
- * (clspmv_coo2dia with alignment 64).  BSR r x r (2 <= r <= 4), sliced
+ *   CLSPMV_VAL_FP16_ROW  FP16 times one power-of-two FP32 scale per row
+ *                        (block row) (2 B)
+ *   CLSPMV_VAL_FP8_SLICE FP8 E4M3 (OCP) times one FP32 scale per slice: 256
+ *                        rows of a diagonal, 64 block rows of a block
+ *                        element, or one 128-entry chunk of the x-tiled
+ *                        layout (1 B)
+ * accumulated in FP32.  DIA: dia_length_aligned must be a multiple of 256
+ * (clspmv_coo2dia with alignment 256).  BSR: width >= block width.  BSR r x r (2 <= r <= 4), sliced
  * layout; delta_cols != 0 stores block columns as int16 steps within a block
  * row when every step fits (else int32; info.method tells: 16 + codec, + 4
- * with int16 columns). */
+ * with int16 columns).  CSR: the LDS x-tiled layout (whose entries pack row
+ * and column in 4 bytes) with encoded values; info.method = 16 + codec. */
 #define CLSPMV_VAL_BF16X2     1
 #define CLSPMV_VAL_FP16_ROW   2
 #define CLSPMV_VAL_FP8_SLICE  3
 int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int codec, clspmv_part_t* p);
 int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int codec, int delta_cols, clspmv_part_t* p);
+int clspmv_part_csr_mixed(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int codec, clspmv_part_t* p);
 int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
 /* Process-wide tuning knobs (no reference counterpart); each affects parts
  * created afterwards:

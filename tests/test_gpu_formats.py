@@ -177,6 +177,21 @@ def test_dia_bdia(ctx, name, make, method):
         _check(coo, y, x, y0, f"BDIA m{method} {name}")
 
 
+@pytest.mark.parametrize("name,make", [("lap2d_128", lambda: M.gen_laplacian2d(128, 128)),
+                                       ("lap2d_129x127", lambda: M.gen_laplacian2d(129, 127)),
+                                       ("stencil27_24", lambda: M.gen_stencil27(24, 5))],
+                         ids=["lap2d_128", "lap2d_129x127", "stencil27_24"])
+@pytest.mark.parametrize("method", [0, 1])
+def test_dia_interior_workgroups(ctx, name, make, method):
+    # enough rows that the four-row kernel has interior workgroups (every
+    # column in x) between boundary ones, and a row count not divisible by 4
+    coo = make()
+    x, y0 = _x_y(coo)
+    y, info = gpu_spmv(ctx, L.FMT_DIA, M.coo2dia(coo), x, y0, method)
+    assert info["kernel"] == ("dia_row4" if method else "dia_row")
+    _check(coo, y, x, y0, f"DIA m{method} {name}")
+
+
 SHAPES = [(bh, bw) for bw in (4, 8) for bh in (1, 2, 4, 8)]
 
 
@@ -209,6 +224,16 @@ def test_bsr_ragged(ctx, method):
     x, y0 = _x_y(coo)
     y, info = gpu_spmv(ctx, L.FMT_BSR, M.coo2bsr(coo, 3, 3), x, y0, method)
     _check(coo, y, x, y0, f"BSR ragged {info['kernel']}")
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16])
+def test_bsr_explicit_lanes(ctx, lanes):
+    # method L = lanes per block row (16 shares its number with no other kind)
+    coo = M.gen_block_rows(400, 3, 9, 60, 21)
+    x, y0 = _x_y(coo)
+    y, info = gpu_spmv(ctx, L.FMT_BSR, M.coo2bsr(coo, 3, 3), x, y0, lanes)
+    assert info["kernel"] == f"bsr33_L{lanes}"
+    _check(coo, y, x, y0, f"BSR {info['kernel']}")
 
 
 def test_repeat_accumulates(ctx):

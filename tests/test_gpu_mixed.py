@@ -33,8 +33,19 @@ def _err(y, coo, x, y0):
     return float(np.max(np.abs(y - y64) / np.maximum(np.maximum(np.abs(y64), S), 1e-30)))
 
 
+def _rect_band(h, w, offsets, seed):
+    """h x w with full diagonals at the given column offsets (col = row + o)."""
+    rng = np.random.default_rng(seed)
+    r = np.concatenate([np.arange(h)[(np.arange(h) + o >= 0) & (np.arange(h) + o < w)] for o in offsets])
+    c = np.concatenate([np.arange(h)[(np.arange(h) + o >= 0) & (np.arange(h) + o < w)] + o for o in offsets])
+    o = np.lexsort((c, r))
+    v = rng.standard_normal(r.size).astype(np.float32)
+    return M.CooMatrix.from_arrays(r[o].astype(np.int32), c[o].astype(np.int32), v, w, h)
+
+
 DIA_CASES = [("lap2d", lambda: M.gen_laplacian2d(64, 48)), ("stencil27", lambda: M.gen_stencil27(20, 3)),
-             ("band_ragged", lambda: M.gen_stencil27(7, 4))]
+             ("band_ragged", lambda: M.gen_stencil27(7, 4)), ("lap2d_ragged", lambda: M.gen_laplacian2d(45, 41)),
+             ("rect_wide", lambda: _rect_band(700, 3000, [-3, 0, 5, 1000, 2299], 17))]
 
 
 @pytest.mark.parametrize("codec", CODECS)
@@ -42,7 +53,7 @@ DIA_CASES = [("lap2d", lambda: M.gen_laplacian2d(64, 48)), ("stencil27", lambda:
 def test_dia_mixed(ctx, codec, name, make):
     coo = make()
     x, y0 = M.gen_vector(coo.width, 1), M.gen_vector(coo.height, 2)
-    part = M.MixedPart(ctx, L.FMT_DIA, M.coo2dia(coo, 64), codec)
+    part = M.MixedPart(ctx, L.FMT_DIA, M.coo2dia(coo, 256), codec)
     assert part.info["method"] == 16 + codec
     e = _err(_run(ctx, part, coo, x, y0), coo, x, y0)
     assert e <= BOUND[codec], (name, codec, e)
@@ -81,4 +92,94 @@ def test_bsr_delta_falls_back_on_wide_steps(ctx):
     assert part.info["method"] == 16 + L.VAL_FP16_ROW and "_d16" not in part.info["kernel"]
     x, y0 = M.gen_vector(coo.width, 5), np.zeros(2, np.float32)
     assert _err(_run(ctx, part, coo, x, y0), coo, x, y0) <= BOUND[L.VAL_FP16_ROW]
+    part.free()
+
+
+def _integer_valued(coo, seed):
+    """The same pattern with values in {-8..8} \\ {0}: exact in BF16 and,
+    after a power-of-two row scale, in FP16."""
+    rng = np.random.default_rng(seed)
+    v = rng.integers(1, 9, coo.nnz) * rng.choice([-1, 1], coo.nnz)
+    return M.CooMatrix.from_arrays(coo.row_id.copy(), coo.col_id.copy(), v.astype(np.float32), coo.width, coo.height)
+
+
+def _wide_long_rows():
+    from tests.test_gpu_xtile import _wide_long_rows as make
+    return make()
+
+
+CSR_CASES = [("uniform_rows", lambda: M.gen_uniform_rows(3000, 17, 3)),
+             ("zipf", lambda: M.gen_zipf_rows(5000, 1.8, 16.0, 3000, 11)),
+             ("wide_ragged_slices", lambda: rand_coo(5000, 100003, 200000, 4)),
+             ("rows_over_block_cap", lambda: M.gen_uniform_rows(40000, 8, 5)),
+             ("empty_rows", lambda: rand_coo(517, 517, 90, 2)),
+             ("wide_long_rows", _wide_long_rows)]
+
+
+@pytest.mark.parametrize("codec", CODECS)
+@pytest.mark.parametrize("name,make", CSR_CASES, ids=[c[0] for c in CSR_CASES])
+def test_csr_mixed_xtile(ctx, codec, name, make):
+    coo = make()
+    x, y0 = M.gen_vector(coo.width, 6), M.gen_vector(coo.height, 7)
+    part = M.MixedPart(ctx, L.FMT_CSR_VECTOR, M.coo2csr(coo), codec)
+    assert part.info["method"] == 16 + codec and part.info["kernel"].startswith("xtile_")
+    e = _err(_run(ctx, part, coo, x, y0), coo, x, y0)
+    assert e <= BOUND[codec], (name, codec, e)
+    part.free()
+
+
+@pytest.mark.parametrize("codec", [L.VAL_BF16X2, L.VAL_FP16_ROW])
+@pytest.mark.parametrize("name,make", CSR_CASES, ids=[c[0] for c in CSR_CASES])
+def test_csr_mixed_exact_values_match_fp32(ctx, codec, name, make):
+    # Values that both codecs hold exactly (FP16 after an exact power-of-two
+    # scale): every product and partial sum is the FP32 path's times a power
+    # of two, so y must be bitwise that of the FP32 x-tiled part.
+    coo = _integer_valued(make(), 13)
+    csr = M.coo2csr(coo)
+    x, y0 = M.gen_vector(coo.width, 8), M.gen_vector(coo.height, 9)
+    mixed = M.MixedPart(ctx, L.FMT_CSR_VECTOR, csr, codec)
+    plain = M.Part(ctx, L.FMT_CSR_VECTOR, csr, 4)
+    ya, yb = _run(ctx, mixed, coo, x, y0), _run(ctx, plain, coo, x, y0)
+    assert np.array_equal(ya.view(np.uint32), yb.view(np.uint32)), (name, codec)
+    mixed.free()
+    plain.free()
+
+
+def test_csr_mixed_stored_bytes(ctx):
+    # 8 bytes per x-tiled entry in FP32 and BF16 hi+lo, 6 with FP16 (+ row
+    # scales), 5 with FP8 (+ one scale per 128-entry chunk)
+    coo = M.gen_uniform_rows(4096, 32, 6)  # 128 chunks of 128 entries, no padding
+    csr = M.coo2csr(coo)
+    plain = M.Part(ctx, L.FMT_CSR_VECTOR, csr, 4)
+    parts = {c: M.MixedPart(ctx, L.FMT_CSR_VECTOR, csr, c) for c in CODECS}
+    base = plain.info["stored_bytes"] - 8 * coo.nnz
+    assert parts[L.VAL_BF16X2].info["stored_bytes"] == plain.info["stored_bytes"]
+    assert parts[L.VAL_FP16_ROW].info["stored_bytes"] == base + 6 * coo.nnz + 4 * coo.height
+    assert parts[L.VAL_FP8_SLICE].info["stored_bytes"] == base + 5 * coo.nnz + 4 * (coo.nnz // 128)
+    for p in list(parts.values()) + [plain]:
+        p.free()
+
+
+def _e4m3(q):
+    """E4M3 (OCP) value of code q from its definition."""
+    s, e, m = q >> 7, (q >> 3) & 15, q & 7
+    v = 2.0 ** (e - 7) * (1 + m / 8) if e else m * 2.0 ** -9
+    return -v if s else v
+
+
+def test_e4m3_decode_every_code(ctx):
+    # A 256-row diagonal (one FP8 slice) whose values are every finite E4M3
+    # code plus 448, the largest, so that the slice scale is exactly 1: with
+    # x = 1, y is the decoded codes themselves, bitwise.
+    assert _e4m3(0x7e) == 448.0 and _e4m3(0x01) == 2.0 ** -9 and _e4m3(0x08) == 2.0 ** -6
+    codes = [q for q in range(256) if q not in (0x7f, 0xff, 0x7e, 0xfe)]
+    n = 256
+    vals = np.full(n, 448.0, np.float32)
+    vals[1:1 + len(codes)] = [_e4m3(q) for q in codes]
+    vals[-1] = -448.0
+    idx = np.arange(n, dtype=np.int32)
+    coo = M.CooMatrix.from_arrays(idx, idx, vals, n, n)
+    part = M.MixedPart(ctx, L.FMT_DIA, M.coo2dia(coo, 256), L.VAL_FP8_SLICE)
+    y = _run(ctx, part, coo, np.ones(n, np.float32), np.zeros(n, np.float32))
+    assert np.array_equal(y, vals)
     part.free()
