@@ -4,15 +4,17 @@
 // A random-column SpMV that reads x from global memory costs one L2 request
 // per nonzero; on MI355X that request rate, not HBM, bounds it.  This layout
 // moves the reuse of x into LDS instead:
-//   * a row block (<= 16384 rows, balanced by nonzeros) is owned by one
-//     1024-thread workgroup, which keeps the block's y partial sums in LDS;
-//   * the columns are cut into 16384-wide slices; for every slice in which
+//   * a row block (<= 32768 rows) is owned by one to eight 1024-thread
+//     workgroups (one per column group, below), each of which keeps the
+//     block's y partial sums in LDS (128 KiB);
+//   * the columns are cut into 8128-wide slices; for every slice in which
 //     the block has entries (a "tile"; a slice with more than 32 chunks is
 //     cut into several tiles) the workgroup stages x[slice] into LDS with
-//     coalesced float4 loads.  Three tiles ahead of use, their x slices and
-//     entry chunks are already in flight in three register sets (the tile
-//     loop is unrolled by three so that each set stays in place), which
-//     covers the HBM/MALL latency that a one-tile prefetch leaves exposed;
+//     coalesced float4 loads.  kXtDepth tiles ahead of use, their x slices
+//     and entry chunks are already in flight in as many register sets (the
+//     tile loop is unrolled by kXtDepth so that each set stays in place),
+//     which covers the HBM/MALL latency that a one-tile prefetch leaves
+//     exposed;
 //   * a tile's entries are sorted by (row, col), packed as
 //     (row_in_block << 16 | col_in_slice) plus the value, and padded to
 //     128-entry chunks.  A wavefront reduces one chunk (2 entries per lane)
@@ -20,17 +22,16 @@
 //     is added to its row by that wavefront alone, the chunk's first segment
 //     (which may continue an earlier chunk) goes to a per-chunk carry that is
 //     added after the next barrier, runs of equal rows in chunk order.
-//   * when x is wider than one XCD's L2 can hold, the slices are split into
-//     G = 2, 4 or 8 column groups of <= 2 MiB of x; workgroup b works on
-//     row block b / G and group b % G, so (workgroups being dispatched
-//     round-robin over the 8 XCDs) every XCD streams only its own group's x
-//     and keeps it in L2.  Each workgroup then writes its partial sums to
-//     partial[group][row] and a second kernel adds the G partials to y in
-//     group order.
+//   * x is staged once per row block (the x traffic is row blocks x x
+//     bytes), so row blocks are as tall as LDS allows; to still give every
+//     CU a workgroup, the slices are split into G <= 8 contiguous column
+//     groups: workgroup b works on row block b / G and group b % G and
+//     writes its partial sums to partial[group][row]; a second kernel adds
+//     the G partials to y in group order.
 // No atomics: the summation order is fixed by the layout, so results are
-// deterministic.  Every workgroup streams all of x it touches through LDS,
-// so the x traffic is (row blocks) x (slice bytes) from L2/MALL, in
-// full cache lines, against one L2 request per nonzero for a gather.
+// deterministic.  Every workgroup streams the x of its group's slices that
+// it touches through LDS, in full cache lines, against one L2 request per
+// nonzero for a gather.
 //
 // Mixed-precision values (the CODEC template argument; FP32 products and
 // sums): BF16 hi + lo words (4 bytes per entry, as FP32), FP16 divided by a
@@ -166,7 +167,7 @@ __device__ __forceinline__ f32x2 xt_decode(typename XtVal<CODEC>::T v, float s) 
 // continues the previous tile's slice) and this wavefront's chunks.
 template <int CODEC>
 struct XtSet {
-    f32x4u x[kXtCols / (4 * kXtThreads)];
+    f32x4u x[kXtXLoads];
     u32x2 e[kXtRounds];
     typename XtVal<CODEC>::T v[kXtRounds];
     float s[kXtRounds];  // FP8 chunk scales
@@ -192,7 +193,8 @@ template <int CODEC>
 __device__ __forceinline__ void xt_fetch_x(XtSet<CODEC>& S, const XtArgs& a, int t, int tid) {
     const int base = (a.tile_slice[t] >> 1) * kXtCols;
 #pragma unroll
-    for (int q = 0; q < kXtCols / (4 * kXtThreads); ++q) S.x[q] = load_x4(a.x, base + (q * kXtThreads + tid) * 4, a.cols);
+    for (int q = 0; q < kXtXLoads; ++q)
+        S.x[q] = load_x4(a.x, base + min(q * kXtThreads + tid, kXtX4 - 1) * 4, a.cols);
 }
 template <int CODEC>
 __device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int t, int wave, int lane) {
@@ -231,14 +233,17 @@ __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, const XtArgs& a, int t,
         const int base = (ts >> 1) * kXtCols;
         if (base + kXtCols <= a.cols) {
 #pragma unroll
-            for (int q = 0; q < kXtCols / (4 * kXtThreads); ++q) {
+            for (int q = 0; q < kXtXLoads; ++q) {
+                const int i = q * kXtThreads + tid;
                 const f32x4u v = S.x[q];
-                x_l4[q * kXtThreads + tid] = make_float4(v.x, v.y, v.z, v.w);
+                if (kXtX4 % kXtThreads == 0 || i < kXtX4) x_l4[i] = make_float4(v.x, v.y, v.z, v.w);
             }
         } else {  // the slice holding the end of x
 #pragma unroll
-            for (int q = 0; q < kXtCols / (4 * kXtThreads); ++q)
-                x_l4[q * kXtThreads + tid] = x_store4(S.x[q], base + (q * kXtThreads + tid) * 4, a.cols);
+            for (int q = 0; q < kXtXLoads; ++q) {
+                const int i = q * kXtThreads + tid;
+                if (kXtX4 % kXtThreads == 0 || i < kXtX4) x_l4[i] = x_store4(S.x[q], base + i * 4, a.cols);
+            }
         }
     }
     const int c0 = a.tile_chunk[tt], c1 = live ? a.tile_chunk[tt + 1] : c0;
@@ -276,7 +281,7 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     // become scalar loads that do not wait on the vector-load queue
     const XtArgs a{blk_row, blk_tile, tile_slice, tile_chunk, idx2, val, vscale, cols, last_chunk, x, y};
     __shared__ float y_l[kXtRows];
-    __shared__ float4 x_l4[kXtCols / 4];
+    __shared__ float4 x_l4[kXtX4];
     __shared__ int c_row[2][kXtWaves];
     __shared__ float c_val[2][kXtWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -292,19 +297,18 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     }
     for (int r = tid; r < nr; r += kXtThreads) y_l[r] = 0.f;
 
-    static_assert(kXtDepth == 3, "the tile loop below is unrolled for three register sets");
-    XtSet<CODEC> A, B, C;
-    xt_fetch_x(A, a, t0, tid);
-    xt_fetch_e(A, a, t0, wave, lane);
-    xt_fetch_x(B, a, min(t0 + 1, t1 - 1), tid);
-    xt_fetch_e(B, a, min(t0 + 1, t1 - 1), wave, lane);
-    xt_fetch_x(C, a, min(t0 + 2, t1 - 1), tid);
-    xt_fetch_e(C, a, min(t0 + 2, t1 - 1), wave, lane);
+    // kXtDepth register sets, indexed only by constants (the loops over them
+    // are unrolled), so they stay in registers
+    XtSet<CODEC> S[kXtDepth];
+#pragma unroll
+    for (int u = 0; u < kXtDepth; ++u) {
+        xt_fetch_x(S[u], a, min(t0 + u, t1 - 1), tid);
+        xt_fetch_e(S[u], a, min(t0 + u, t1 - 1), wave, lane);
+    }
     XtState st{0, false};
-    for (int t = t0; t < t1; t += 3) {
-        xt_step(A, a, t, t1, y_l, x_l4, c_row, c_val, st, tid, wave, lane);
-        xt_step(B, a, t + 1, t1, y_l, x_l4, c_row, c_val, st, tid, wave, lane);
-        xt_step(C, a, t + 2, t1, y_l, x_l4, c_row, c_val, st, tid, wave, lane);
+    for (int t = t0; t < t1; t += kXtDepth) {
+#pragma unroll
+        for (int u = 0; u < kXtDepth; ++u) xt_step(S[u], a, t + u, t1, y_l, x_l4, c_row, c_val, st, tid, wave, lane);
     }
     __syncthreads();
     if (st.pending && tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);

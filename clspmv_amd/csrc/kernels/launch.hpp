@@ -108,14 +108,18 @@ constexpr int kCodecFp8Slice = 3;
 // col_in_slice) in chunks of kXtChunk; padding entries carry row kXtPadRow.
 constexpr int kXtThreads = 1024;
 constexpr int kXtWaves = kXtThreads / 64;
-constexpr int kXtRows = 16384;
-constexpr int kXtCols = 16384;
+// LDS: y partial sums of kXtRows rows (128 KiB) + one x slice of kXtCols
+// columns (31.75 KiB) + 2 x 16 carry slots (256 B) = 160 KiB, all of it.
+constexpr int kXtRows = 32768;
+constexpr int kXtCols = 8128;
+constexpr int kXtX4 = kXtCols / 4;                                  // float4 per slice
+constexpr int kXtXLoads = (kXtX4 + kXtThreads - 1) / kXtThreads;    // float4 per thread
 constexpr int kXtChunk = 128;
 constexpr int kXtRounds = 2;                       // chunk rounds per tile held in registers
 constexpr int kXtTileChunks = kXtRounds * kXtWaves;  // a slice with more chunks is split
 constexpr int kXtDepth = 3;                        // tiles prefetched ahead
 constexpr unsigned kXtPadRow = 0xFFFFu;
-constexpr int kXtGroupSlices = 32;  // column slices per group: 2 MiB of x, resident in one XCD's L2
+constexpr int kXtMaxGroups = 8;    // column groups per row block at most
 struct DevXtile {
     const int* blk_row;     // nblocks + 1 row boundaries
     const int* blk_tile;    // nblocks + 1 tile ranges
@@ -126,7 +130,7 @@ struct DevXtile {
     const float* vscale;    // FP16: rows row scales; FP8: nchunks chunk scales
     float* partial;         // groups * rows partial sums when groups > 1
     int nblocks;            // workgroups = row blocks * groups
-    int groups;             // column groups (1, 2, 4 or 8); workgroup b works on group b % groups
+    int groups;             // column groups (1..8); workgroup b works on group b % groups
     int ntiles, nchunks, cols, rows;
     int codec;              // 0 (FP32) or a kCodec* value
 };

@@ -577,17 +577,19 @@ int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int
     return CLSPMV_OK;
 }
 
-int g_xt_blocks = 256;     // target row blocks: one per CU (tunable: "xtile_blocks")
-// Column groups at most (tunable: "xtile_max_groups", 1, 2, 4 or 8).  One
-// group is the default: on a 4M x 4M uniform matrix grouping by XCD was
-// slower at every density (the partial-sum pass and the shorter workgroups
-// cost more than the L2 hits gain).
-int g_xt_max_groups = 1;
+int g_xt_blocks = 256;  // target workgroups: one per CU (tunable: "xtile_blocks")
+// Column groups per row block at most (tunable: "xtile_max_groups", 1..8).
+int g_xt_max_groups = kXtMaxGroups;
 
-// Builds the LDS x-tiled layout (DevXtile) from row lists.  Row blocks are
-// balanced by entry count (at most kXtRows rows each); within a block the
-// entries are bucketed by column slice in row order, so each tile is sorted
-// by (row, col) when the rows are.
+// Builds the LDS x-tiled layout (DevXtile) from row lists.  Every row block
+// stages the x it touches once, so the x traffic is (row blocks) x (x
+// bytes): the blocks are as tall as LDS allows (kXtRows), at least W / 8 of
+// them (W = target workgroups), balanced by entry count; each block is then
+// split into G = W / blocks (<= 8) column groups so that the launch still
+// has about W workgroups.  When balancing by entries would need more
+// blocks than that minimum and so lower G, the rows are cut evenly instead.
+// Within a block the entries are bucketed by column slice in row order, so
+// each tile is sorted by (row, col) when the rows are.
 // `codec` (0 or a kCodec* value) encodes the values: FP16 divided by a
 // power-of-two scale per row, FP8 with one scale per 128-entry chunk.
 int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
@@ -598,8 +600,12 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
         return CLSPMV_E_UNSUPPORTED;
     }
     const int ns = std::max(1, (cols + kXtCols - 1) / kXtCols);
+    const int W = std::max(1, g_xt_blocks);
+    const int max_g = std::max(1, std::min({g_xt_max_groups, kXtMaxGroups, ns}));
+    const long long nb_rows = std::max(1, (rows + kXtRows - 1) / kXtRows);
+    const long long nb_target = std::max(nb_rows, (long long)(W + max_g - 1) / max_g);
     std::vector<int> brow{0};
-    const long long target = std::max<long long>(1, (nnz + g_xt_blocks - 1) / g_xt_blocks);
+    const long long target = std::max<long long>(1, (nnz + nb_target - 1) / nb_target);
     for (int r = 0; r < rows;) {
         const int rb = r;
         long long n = 0;
@@ -608,6 +614,14 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
             ++r;
         } while (r < rows && r - rb < kXtRows && n < target);
         brow.push_back(r);
+    }
+    auto groups_for = [&](long long nb) { return (int)std::max(1LL, std::min((long long)max_g, W / nb)); };
+    if ((long long)brow.size() - 1 > nb_target && groups_for((long long)brow.size() - 1) < groups_for(nb_target)) {
+        // even rows: nb_target blocks of at most kXtRows rows
+        brow.assign(1, 0);
+        const long long per = (rows + nb_target - 1) / nb_target;
+        for (long long r = per; r < rows; r += per) brow.push_back((int)r);
+        brow.push_back(rows);
     }
     const int nb = (int)brow.size() - 1;
     // per block: (slice, entries) of its nonempty slices
@@ -623,11 +637,9 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
                 if (cnt[(size_t)s]) bt[(size_t)b].push_back({s, cnt[(size_t)s]});
         }
     }
-    // Column groups: G (a power of two <= 8) groups of at most
-    // kXtGroupSlices slices, when x is wider than that.  Workgroup
+    // Column groups: G contiguous ranges of slices; workgroup
     // b = block * G + group.
-    int G = 1;
-    while (G < g_xt_max_groups && (long long)G * kXtGroupSlices < ns) G *= 2;
+    const int G = rows ? groups_for(nb) : 1;
     const int spg = (ns + G - 1) / G;
     // A slice's chunks are cut into tiles of at most kXtTileChunks; the
     // later pieces are marked as continuing the slice (x stays in LDS).
@@ -1382,7 +1394,7 @@ int clspmv_set_option(const char* name, long long value) {
     if (!name) return CLSPMV_E_INVALID;
     if (std::strcmp(name, "cocktail_merge_flat") == 0) return clspmv_cocktail_option(name, value);
     if (std::strcmp(name, "xtile_max_groups") == 0) {
-        if (value != 1 && value != 2 && value != 4 && value != 8) return CLSPMV_E_INVALID;
+        if (value < 1 || value > kXtMaxGroups) return CLSPMV_E_INVALID;
         g_xt_max_groups = (int)value;
         return CLSPMV_OK;
     }

@@ -322,9 +322,10 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
  *            per block row
  * "Column-sliced" cuts the columns into slices whose part of x (<= 2 MiB by
  * default) stays in one XCD's L2; one launch per slice adds into y.
- * "LDS x-tiled" gives each 1024-thread workgroup a block of <= 16384 rows
- * whose partial sums stay in LDS and streams x through LDS in 16384-column
- * slices (for wide x with scattered columns; needs width >= 4).  The
+ * "LDS x-tiled" gives each 1024-thread workgroup a block of <= 32768 rows
+ * whose partial sums stay in LDS and streams x through LDS in 8128-column
+ * slices, a block's slices split into up to 8 column groups (for wide x with
+ * scattered columns; needs width >= 4).  The
  * reference's method ids (e.g. texture variants) map onto these. */
 /* Mixed-precision parts (no counterpart in the reference; for callers that
  * trade value precision for bytes per nonzero): values stored as
@@ -352,11 +353,11 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  * created afterwards:
  *   "col_slice_bytes"   x bytes per column slice of the column-sliced
  *                       layouts (default 2 MiB);
- *   "xtile_blocks"      target row blocks of the x-tiled layout (default
+ *   "xtile_blocks"      target workgroups of the x-tiled layout (default
  *                       256, one per CU);
- *   "xtile_max_groups"  1, 2, 4 or 8 column groups of the x-tiled layout
- *                       (default 1; > 1 assigns each XCD one group of x and
- *                       adds a partial-sum pass);
+ *   "xtile_max_groups"  column groups per row block of the x-tiled layout
+ *                       at most, 1..8 (default 8; with > 1 group a
+ *                       partial-sum pass adds the groups' sums to y);
  *   "cocktail_merge_flat"  1 (default): clspmv_init_mat_kernels runs the
  *                       flat parts (ELL, SELL, CSR, COO) of a partition as
  *                       one merged row-list part; 0: one part each. */

@@ -1,7 +1,7 @@
 """GPU parity of the LDS x-tiled layout (method 4 of the ELL, SELL, CSR and
 COO parts) against the CPU oracle, on shapes that exercise its edges: several
 column slices with a ragged last one, row blocks split by entry count and by
-the 16384-row cap, tiles with one and with many rounds of chunks (one row
+the 32768-row cap, tiles with one and with many rounds of chunks (one row
 block forced), rows longer than a chunk, empty rows and empty blocks, and
 tiles whose entry count is an exact multiple of the chunk (no padding).
 Results must also be bitwise repeatable."""
@@ -54,11 +54,11 @@ CASES = [
     ("rand_sparse_empty_rows", lambda: rand_coo(517, 517, 90, 2)),
     ("one_row", lambda: rand_coo(1, 40, 25, 3)),
     ("wide_ragged_slices", lambda: rand_coo(5000, 100003, 200000, 4)),
-    ("rows_over_block_cap", lambda: M.gen_uniform_rows(40000, 8, 5)),
+    ("rows_over_block_cap", lambda: M.gen_uniform_rows(70000, 8, 5)),
     ("exact_chunks", lambda: M.gen_uniform_rows(4096, 32, 6)),
     ("wide_long_rows", _wide_long_rows),
     ("zipf_wide", lambda: M.gen_zipf_rows(60000, 1.8, 16.0, 20000, 12)),
-    # column groups: 74 slices -> 4 groups; 306 slices -> 8 groups, ragged tail
+    # wide x: many slices split into column groups, ragged last slice
     ("groups4", lambda: _rand_wide(20000, 1200000, 300000, 14)),
     ("groups8", lambda: _rand_wide(3000, 5000001, 100000, 15)),
 ]
@@ -92,7 +92,11 @@ def xt_blocks():
 def xt_groups():
     L.check(L.lib().clspmv_set_option(b"xtile_max_groups", 8), "set_option")
     yield
-    L.check(L.lib().clspmv_set_option(b"xtile_max_groups", 1), "set_option")
+    L.check(L.lib().clspmv_set_option(b"xtile_max_groups", 8), "set_option")
+
+
+def _groups(info):
+    return int(info["kernel"].rsplit("_G", 1)[1])
 
 
 @pytest.mark.parametrize("name,make", CASES, ids=IDS)
@@ -106,7 +110,7 @@ def test_xtile(ctx, xt_groups, name, make, fmt):
     y, info = gpu_spmv(ctx, fmt, _convert(fmt, coo), x, y0, XT)
     assert info["method"] == XT and info["kernel"].startswith("xtile"), info
     if name.startswith("groups"):
-        assert info["kernel"].endswith("_G" + name[-1]), info
+        assert _groups(info) > 1, info
     _check(coo, y, x, y0, f"xtile fmt{fmt} {name}")
 
 
@@ -123,7 +127,8 @@ def test_xtile_block_counts(ctx, xt_blocks, nblocks, name, groups):
     try:
         y, info = gpu_spmv(ctx, L.FMT_CSR_VECTOR, M.coo2csr(coo), x, y0, XT)
     finally:
-        L.check(L.lib().clspmv_set_option(b"xtile_max_groups", 1), "set_option")
+        L.check(L.lib().clspmv_set_option(b"xtile_max_groups", 8), "set_option")
+    assert _groups(info) <= groups
     _check(coo, y, x, y0, f"xtile blocks={nblocks} {name} {info['kernel']}")
 
 
@@ -153,3 +158,16 @@ def test_xtile_empty(ctx):
     y0 = M.gen_vector(50, 1)
     y, _ = gpu_spmv(ctx, L.FMT_CSR_VECTOR, M.coo2csr(coo), M.gen_vector(100, 2), y0, XT)
     assert np.array_equal(y, y0)
+
+
+def test_xtile_even_rows_when_balancing_costs_groups(ctx, xt_blocks):
+    # 3 x 32768 Zipf rows with 16 target workgroups: balancing by entries
+    # would need a fourth row block (and only 4 groups each); the rows are
+    # cut evenly into 3 full blocks of 5 column groups instead
+    coo = M.gen_zipf_rows(3 * 32768, 1.8, 16.0, 20000, 31)
+    xt_blocks(16)
+    x = M.gen_vector(coo.width, 9)
+    y0 = M.gen_vector(coo.height, 10)
+    y, info = gpu_spmv(ctx, L.FMT_CSR_VECTOR, M.coo2csr(coo), x, y0, XT)
+    assert info["kernel"] == "xtile_B3_G5", info
+    _check(coo, y, x, y0, f"xtile even rows {info['kernel']}")
