@@ -170,12 +170,13 @@ def _errors(y, y64, s):
 
 
 def mixed_c3(ctx, args, work, per):
-    """Mixed precision on the c1 matrices (c3): each matrix in its c1 layout
-    with values stored as BF16 hi+lo, FP16 with a row scale or FP8 E4M3 with
-    a slice scale (DIA and x-tiled CSR parts; BSR with int16 delta block
-    columns), FP32 accumulation.  Per matrix and codec: kernel time (HIP
-    events), stored bytes, effective nonzeros/s, speedup over the FP32 part,
-    and the error of one y = A x against the FP64 product."""
+    """Mixed precision (c3) on the c1 matrices and the c2 composite: each c1
+    matrix in its c1 layout with values stored as BF16 hi+lo, FP16 with a
+    row scale or FP8 E4M3 with a slice scale (DIA and x-tiled CSR parts; BSR
+    with int16 delta block columns), FP32 accumulation; c2 as one x-tiled
+    CSR part against its FP32 Cocktail partition.  Per matrix and codec:
+    kernel time (HIP events), stored bytes, effective nonzeros/s, speedup
+    over FP32, and the error of one y = A x against the FP64 product."""
     import numpy as np
 
     from clspmv_amd import _lib as L
@@ -213,20 +214,61 @@ def mixed_c3(ctx, args, work, per):
             fmt, mat = L.FMT_BSR, M.coo2bsr(coo, 3, 3)
         else:
             fmt, mat = L.FMT_CSR_VECTOR, M.coo2csr(coo)
-        for codec, label in ((L.VAL_BF16X2, "bf16x2"), (L.VAL_FP16_ROW, "fp16row"), (L.VAL_FP8_SLICE, "fp8slice")):
-            part = M.MixedPart(ctx, fmt, mat, codec, True)
-            ms = time_part(part, w["dx"], w["dy"])
-            e_max, e_norm = _errors(run_once(part, w["dx"], coo.height), y64, srow)
-            rec[label] = {"kernel": part.info["kernel"], "ms": round(ms, 5),
-                          "stored_bytes": int(part.info["stored_bytes"]),
-                          "eff_gnnz_per_s": round(coo.nnz / (ms * 1e-3) / 1e9, 1),
-                          "speedup_vs_fp32": round(ms32 / ms, 3), "max_rel_err": e_max, "norm_rel_err": e_norm}
-            log(f"c3 {name} {part.info['kernel']}: {ms * 1e3:.1f} us ({ms32 / ms:.2f}x FP32), "
-                f"{part.info['stored_bytes'] / 1e6:.1f} MB, max rel err {e_max:.2e}, norm rel err {e_norm:.2e}")
-            part.free()
+        _mixed_codecs(ctx, M, L, rec, fmt, mat, coo, w, ms32, y64, srow, time_part, run_once)
         mat.free()
         out[name] = rec
+    if args.no_c2:
+        return out
+    # c2: the Cocktail composite; FP32 baseline = the Cocktail object with
+    # the MI355X tables, mixed = the whole matrix as an x-tiled CSR part
+    from clspmv_amd import cocktail as CK
+    from clspmv_amd import workloads as W
+
+    coo = W.make_coo("c2", args.scale)
+    x = M.gen_vector(coo.width, 7)
+    mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(CK.MI355X_TABLES))
+    gpu = CK.GpuCocktail(ctx, mat, x, np.zeros(coo.height, np.float32))
+    gpu.do_spmv(2)
+    ctx.sync()
+    ctx.time_begin()
+    gpu.do_spmv(k)
+    ms32 = ctx.time_end() / k
+    gpu.result_from_cpu(np.zeros(coo.height, np.float32))
+    gpu.do_spmv(1)
+    y64, srow = _fp64_reference(coo, x)
+    e_max, e_norm = _errors(gpu.result_to_cpu(), y64, srow)
+    stored = sum(p["stored_bytes"] for p in gpu.part_infos())
+    rec = {"fp32": {"kernel": "Cocktail: " + "+".join(p["kernel"] for p in gpu.part_infos()), "ms": round(ms32, 5),
+                    "stored_bytes": int(stored), "eff_gnnz_per_s": round(coo.nnz / (ms32 * 1e-3) / 1e9, 1),
+                    "max_rel_err": e_max, "norm_rel_err": e_norm}}
+    gpu.free()
+    mat.free()
+    dx = M.DeviceArray.from_numpy(ctx, x)
+    dy = M.DeviceArray.from_numpy(ctx, np.zeros(coo.height, np.float32))
+    csr = M.coo2csr(coo)
+    _mixed_codecs(ctx, M, L, rec, L.FMT_CSR_VECTOR, csr, coo, {"dx": dx, "dy": dy}, ms32, y64, srow, time_part,
+                  run_once, name="c2")
+    csr.free()
+    dx.free()
+    dy.free()
+    coo.free()
+    out["c2"] = rec
     return out
+
+
+def _mixed_codecs(ctx, M, L, rec, fmt, mat, coo, w, ms32, y64, srow, time_part, run_once, name=None):
+    """Adds one record per value codec to rec (time, bytes, speedup, errors)."""
+    for codec, label in ((L.VAL_BF16X2, "bf16x2"), (L.VAL_FP16_ROW, "fp16row"), (L.VAL_FP8_SLICE, "fp8slice")):
+        part = M.MixedPart(ctx, fmt, mat, codec, True)
+        ms = time_part(part, w["dx"], w["dy"])
+        e_max, e_norm = _errors(run_once(part, w["dx"], coo.height), y64, srow)
+        rec[label] = {"kernel": part.info["kernel"], "ms": round(ms, 5),
+                      "stored_bytes": int(part.info["stored_bytes"]),
+                      "eff_gnnz_per_s": round(coo.nnz / (ms * 1e-3) / 1e9, 1),
+                      "speedup_vs_fp32": round(ms32 / ms, 3), "max_rel_err": e_max, "norm_rel_err": e_norm}
+        log(f"c3 {name or w.get('name')} {part.info['kernel']}: {ms * 1e3:.1f} us ({ms32 / ms:.2f}x FP32), "
+            f"{part.info['stored_bytes'] / 1e6:.1f} MB, max rel err {e_max:.2e}, norm rel err {e_norm:.2e}")
+        part.free()
 
 
 def batched_c4(ctx, args):
@@ -273,9 +315,11 @@ def batched_c4(ctx, args):
                       "launch": "one hipGraph replay"}
     b.free()
 
-    def per_matrix(choose):
+    def per_matrix(choose, only=None):
         parts = []
-        for (kind, coo, k), csr in zip(mats, csrs):
+        for i, ((kind, coo, k), csr) in enumerate(zip(mats, csrs)):
+            if only is not None and i not in only:
+                continue
             fmt, meth, mat = choose(coo, csr)
             part = M.Part(ctx, fmt, mat, meth)
             if mat is not csr and mat is not coo:
@@ -297,9 +341,38 @@ def batched_c4(ctx, args):
             y.free()
         return ms
 
+    # the same split by k (each class alone), so the source of the gain is visible
+    by_k = {}
+    for kk in (1, 4, 16):
+        idx = [i for i, k in enumerate(ks) if k == kk]
+        if not idx:
+            continue
+        sub = B.Batch(ctx, [csrs[i] for i in idx], [kk] * len(idx))
+        for j, i in enumerate(idx):
+            sub.set_x(j, np.ones((mats[i][1].width, kk), np.float32))
+        sub.capture()
+        sub.run(1)
+        ctx.sync()
+        ctx.time_begin()
+        sub.run(reps)
+        ms_bk = ctx.time_end() / reps
+        sub.free()
+        nz = sum(mats[i][1].nnz for i in idx)
+        by_k[str(kk)] = {"matrices": len(idx), "nnz_x_rhs": int(nz * kk), "batched_ms": round(ms_bk, 4),
+                         "batched_eff_gnnz_per_s": round(nz * kk / (ms_bk * 1e-3) / 1e9, 1)}
+
     ms_p = per_matrix(lambda coo, csr: (L.FMT_CSR_VECTOR, 1, csr))
     out["plain"] = {"ms": round(ms_p, 4), "eff_gnnz_per_s": round(nnz_rhs / (ms_p * 1e-3) / 1e9, 1),
                     "kernel": "csr_adaptive, k launches per matrix"}
+    for kk, rec in by_k.items():
+        idx = {i for i, k in enumerate(ks) if k == int(kk)}
+        ms_pk = per_matrix(lambda coo, csr: (L.FMT_CSR_VECTOR, 1, csr), idx)
+        rec["plain_ms"] = round(ms_pk, 4)
+        rec["plain_eff_gnnz_per_s"] = round(rec["nnz_x_rhs"] / (ms_pk * 1e-3) / 1e9, 1)
+        rec["batched_vs_plain"] = round(ms_pk / rec["batched_ms"], 3)
+        log(f"c4 k={kk}: {rec['matrices']} matrices, plain {ms_pk:.2f} ms, batched {rec['batched_ms']:.2f} ms "
+            f"({rec['batched_vs_plain']}x)")
+    out["by_k"] = by_k
     sel = S.Selector.load_default(ctx)
 
     def pick(coo, csr):
