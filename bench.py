@@ -487,6 +487,23 @@ def main():
         }
         log(f"  {w['name']} {info['kernel']}: {ms * 1e3:.1f} us  {per[w['name']]['gbps']} GB/s "
             f"({per[w['name']]['pct_hbm_peak']}% of 8 TB/s)")
+    # measured copy bandwidth (1 GiB device-to-device copy kernel): the
+    # achievable streaming rate beside the published 8 TB/s peak
+    import ctypes as C
+
+    from clspmv_amd import _lib as L
+
+    nb = 1 << 30
+    src, dst = M.DeviceArray(ctx, nb // 4), M.DeviceArray(ctx, nb // 4)
+    g = C.c_float()
+    L.check(L.lib().clspmv_probe_copy(ctx.h, src.ptr, dst.ptr, nb, 10, C.byref(g)), "probe_copy")
+    src.free()
+    dst.free()
+    copy_gbps = round(g.value, 1)
+    log(f"copy kernel: {copy_gbps} GB/s (read + write of 1 GiB)")
+    for n in per:
+        per[n]["pct_of_copy"] = round(100.0 * per[n]["gbps"] / copy_gbps, 1)
+
     dom = max(per, key=lambda n: per[n]["ms"])
     d = per[dom]
     achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9
@@ -535,6 +552,7 @@ def main():
             "bytes_alg_per_step": int(bytes_alg),
         },
         "hbm_gbps": round(gbps, 1),
+        "copy_gbps": copy_gbps,
         "pct_hbm_peak": round(100.0 * gbps / HBM_PEAK_GBPS, 2),
         "per_matrix": per,
         "roofline": roofline,
