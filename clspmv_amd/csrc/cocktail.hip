@@ -45,17 +45,17 @@ constexpr int kXtile = 4;  // method number of the x-tiled layout
 // fitted on MI355X: gather ~ 26 us per 2 MiB column slice + 4.65 ps per
 // entry (uniform random columns, 4M x 4M); x-tiled ~ the larger of its x
 // staging (every row block -- at most 32768 rows, at least 32 blocks --
-// stages each 8128-column slice it touches, at ~4.1 TB/s) and 3.9 ps per
+// stages each 8064-column slice it touches, at ~4.1 TB/s) and 3.9 ps per
 // entry (4M x 32 uniform: 520 us), plus the pass adding the column groups'
 // partial sums.
 bool wide_x(int cols) { return 4.0 * cols > 4.0 * 1024 * 1024; }
 bool xtile_pays(long long rows, long long cols, long long nnz) {
     if (nnz <= 0 || cols < 4) return false;
-    const double ns = std::ceil(cols / 8128.0);
+    const double ns = std::ceil(cols / 8064.0);
     const double nb = std::max(32.0, std::ceil(rows / 32768.0));
     const double groups = std::max(1.0, std::min(8.0, std::floor(256.0 / nb)));
     const double per_block = nnz / nb;
-    const double staged = nb * ns * (1.0 - std::exp(-per_block / ns)) * 32512.0;
+    const double staged = nb * ns * (1.0 - std::exp(-per_block / ns)) * 32256.0;
     const double t_xt = std::max(staged / 4.1e12, 3.9e-12 * nnz) + (groups > 1 ? (groups + 2) * rows * 4.0 / 5e12 : 0);
     const double t_gather = 26e-6 * std::ceil(4.0 * cols / (2.0 * 1024 * 1024)) + 4.65e-12 * nnz;
     return t_xt < t_gather;

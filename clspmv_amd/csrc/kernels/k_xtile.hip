@@ -7,7 +7,7 @@
 //   * a row block (<= 32768 rows) is owned by one to eight 1024-thread
 //     workgroups (one per column group, below), each of which keeps the
 //     block's y partial sums in LDS (128 KiB);
-//   * the columns are cut into 8128-wide slices; for every slice in which
+//   * the columns are cut into 8064-wide slices; for every slice in which
 //     the block has entries (a "tile"; a slice with more than 32 chunks is
 //     cut into several tiles) the workgroup stages x[slice] into LDS with
 //     coalesced float4 loads.  kXtDepth tiles ahead of use, their x slices
@@ -78,23 +78,27 @@ __device__ __forceinline__ void xt_apply_carry(const int* cr, const float* cv, f
     y_l[r] = s;
 }
 
-// Emits the sum of a finished segment: the chunk's first row goes to the
-// chunk's carry slot, other rows are added to y_l (padding is dropped).
-__device__ __forceinline__ void xt_emit(int row, float v, int first_row, float* y_l, int* cr, float* cv, int wave) {
-    if (row == first_row) {
-        cr[wave] = row;
-        cv[wave] = v;
-    } else if (row != (int)kXtPadRow) {
-        y_l[row] += v;
-    }
+// Adds one emitted segment sum to y_l -- unless it belongs to the chunk's
+// first row (that one goes to the chunk's carry slot) or to padding -- with
+// no branch: a lane that does not emit here rewrites its own dummy slot
+// instead (the dummies' values are never used; plain loads and stores, so
+// waves touching the same dummy do not serialise as atomics would).  Within
+// one round a row is added to by one lane of one wave only.
+__device__ __forceinline__ void xt_add(bool on, int row, float v, int first_row, float* y_l, float* dummy, int lane) {
+    const bool direct = on && row != first_row && row != (int)kXtPadRow;
+    float* p = direct ? &y_l[row] : &dummy[lane];
+    *p += v;
 }
 
-// One wavefront reduces one 128-entry chunk (2 entries per lane).  Writes
-// y_l for segments that start inside the chunk and the chunk's carry slot
-// for its first row.  Each segment is emitted once, by the lane holding its
-// last entry.
-__device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, float* y_l, int* cr, float* cv,
-                                         int wave, int lane) {
+// One wavefront reduces one 128-entry chunk (2 entries per lane).  Adds the
+// sums of segments that start inside the chunk to y_l and writes the
+// chunk's carry slot for its first row.  Each segment is emitted once, by
+// the lane holding its last entry: at most two emit sites per lane in the
+// common case, three in the general one, all evaluated without branches;
+// the first row's sum (exactly one site emits it) is fetched from its lane
+// with a ballot and a readlane.
+__device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, float* y_l, float* dummy, int* cr,
+                                         float* cv, int wave, int lane) {
     const int ra = (int)(e.x >> 16), rb = (int)(e.y >> 16);
     const float pa = v.x * x_l[e.x & 0xFFFFu];
     const float pb = v.y * x_l[e.y & 0xFFFFu];
@@ -111,36 +115,50 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
     const bool through = cont && !brk;                        // the whole lane is inside that segment
     const unsigned long long tmask = __ballot(through);
     const unsigned long long cmask = __ballot(cont);
+    // emit sites: (on, row, sum); the third only in the general case
+    bool on1, on2, on3 = false;
+    int row1, row2, row3 = 0;
+    float s1, s2, s3 = 0.f;
     // A segment spans three or more lanes exactly when some lane L lies
     // inside it (through) and lane L + 1 continues it.
     if ((tmask & (cmask >> 1)) == 0) {
         // Every segment spans at most two lanes: one neighbour exchange
         // completes it (the common case for short rows).
-        if (brk) xt_emit(head_row, head_sum + (cont ? prev_tail_sum : 0.f), first_row, y_l, cr, cv, wave);
         const bool next_cont = lane < 63 && ((cmask >> (lane + 1)) & 1ull);
-        if (!next_cont)
-            xt_emit(tail_row, tail_sum + (through ? prev_tail_sum : 0.f), first_row, y_l, cr, cv, wave);
-        return;
-    }
-
-    // General case: segmented inclusive scan of the lane tails.
-    int flag = (lane == 0) || brk || (prev_tail_row != tail_row);
-    float cval = tail_sum;
+        on1 = brk, row1 = head_row, s1 = head_sum + (cont ? prev_tail_sum : 0.f);
+        on2 = !next_cont, row2 = tail_row, s2 = tail_sum + (through ? prev_tail_sum : 0.f);
+        xt_add(on1, row1, s1, first_row, y_l, dummy, lane);
+        xt_add(on2, row2, s2, first_row, y_l, dummy, lane);
+    } else {
+        // General case: segmented inclusive scan of the lane tails.
+        int flag = (lane == 0) || brk || (prev_tail_row != tail_row);
+        float cval = tail_sum;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const float up_v = __shfl_up(cval, d, 64);
-        const int up_f = __shfl_up(flag, d, 64);
-        if (lane >= d && !flag) {
-            cval += up_v;
-            flag = up_f;
+        for (int d = 1; d < 64; d <<= 1) {
+            const float up_v = __shfl_up(cval, d, 64);
+            const int up_f = __shfl_up(flag, d, 64);
+            const bool take = lane >= d && !flag;
+            cval = take ? cval + up_v : cval;
+            flag = take ? up_f : flag;
         }
+        const float prev_c = __shfl_up(cval, 1, 64);
+        // a segment that ended exactly before this lane
+        const bool boundary_end = lane > 0 && prev_tail_row != head_row;
+        on1 = boundary_end, row1 = prev_tail_row, s1 = prev_c;
+        on2 = brk, row2 = head_row, s2 = head_sum + ((lane > 0 && !boundary_end) ? prev_c : 0.f);
+        on3 = lane == 63, row3 = tail_row, s3 = cval;
+        xt_add(on1, row1, s1, first_row, y_l, dummy, lane);
+        xt_add(on2, row2, s2, first_row, y_l, dummy, lane);
+        xt_add(on3, row3, s3, first_row, y_l, dummy, lane);
     }
-    const float prev_c = __shfl_up(cval, 1, 64);
-    // a segment that ended exactly before this lane
-    const bool boundary_end = lane > 0 && prev_tail_row != head_row;
-    if (boundary_end) xt_emit(prev_tail_row, prev_c, first_row, y_l, cr, cv, wave);
-    if (brk) xt_emit(head_row, head_sum + ((lane > 0 && !boundary_end) ? prev_c : 0.f), first_row, y_l, cr, cv, wave);
-    if (lane == 63) xt_emit(tail_row, cval, first_row, y_l, cr, cv, wave);
+    const bool f1 = on1 && row1 == first_row, f2 = on2 && row2 == first_row, f3 = on3 && row3 == first_row;
+    const float fv = f1 ? s1 : (f2 ? s2 : s3);
+    const int src = __builtin_ctzll(__ballot(f1 || f2 || f3));  // exactly one lane emits the first row
+    const float first_sum = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fv), src));
+    if (lane == 0) {
+        cr[wave] = first_row;
+        cv[wave] = first_sum;
+    }
 }
 
 // Per-lane value storage of one chunk (two entries) for each codec.
@@ -221,8 +239,8 @@ struct XtState {
 // issues every load on every path.
 template <int CODEC>
 __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, const XtArgs& a, int t, int t1, float* y_l, float4* x_l4,
-                                        int (*c_row)[kXtWaves], float (*c_val)[kXtWaves], XtState& st, int tid,
-                                        int wave, int lane) {
+                                        float* dummy, int (*c_row)[kXtWaves], float (*c_val)[kXtWaves], XtState& st,
+                                        int tid, int wave, int lane) {
     __syncthreads();  // previous tile done with x_l; its carry slots are written
     if (st.pending && tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
     st.pending = false;
@@ -260,8 +278,8 @@ __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, const XtArgs& a, int t,
                 if (tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
             }
             if (cb + wave < c1)
-                xt_chunk(S.e[k], xt_decode<CODEC>(S.v[k], S.s[k]), x_l, y_l, c_row[st.par], c_val[st.par], wave,
-                         lane);
+                xt_chunk(S.e[k], xt_decode<CODEC>(S.v[k], S.s[k]), x_l, y_l, dummy, c_row[st.par], c_val[st.par],
+                         wave, lane);
             else if (lane == 0)
                 c_row[st.par][wave] = -1;
             st.par ^= 1;
@@ -284,6 +302,7 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     __shared__ float4 x_l4[kXtX4];
     __shared__ int c_row[2][kXtWaves];
     __shared__ float c_val[2][kXtWaves];
+    __shared__ float dummy[64];  // one per lane: where a lane's non-emits add 0
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = blockIdx.x;
     const int rb = b / groups;  // row block; column group b % groups
@@ -308,7 +327,8 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     XtState st{0, false};
     for (int t = t0; t < t1; t += kXtDepth) {
 #pragma unroll
-        for (int u = 0; u < kXtDepth; ++u) xt_step(S[u], a, t + u, t1, y_l, x_l4, c_row, c_val, st, tid, wave, lane);
+        for (int u = 0; u < kXtDepth; ++u)
+            xt_step(S[u], a, t + u, t1, y_l, x_l4, dummy, c_row, c_val, st, tid, wave, lane);
     }
     __syncthreads();
     if (st.pending && tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);

@@ -109,9 +109,10 @@ constexpr int kCodecFp8Slice = 3;
 constexpr int kXtThreads = 1024;
 constexpr int kXtWaves = kXtThreads / 64;
 // LDS: y partial sums of kXtRows rows (128 KiB) + one x slice of kXtCols
-// columns (31.75 KiB) + 2 x 16 carry slots (256 B) = 160 KiB, all of it.
+// columns (31.5 KiB) + 2 x 16 carry slots (256 B) + 64 dummy slots (256 B)
+// = 160 KiB, all of it.
 constexpr int kXtRows = 32768;
-constexpr int kXtCols = 8128;
+constexpr int kXtCols = 8064;
 constexpr int kXtX4 = kXtCols / 4;                                  // float4 per slice
 constexpr int kXtXLoads = (kXtX4 + kXtThreads - 1) / kXtThreads;    // float4 per thread
 constexpr int kXtChunk = 128;

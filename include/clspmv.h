@@ -323,7 +323,7 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
  * "Column-sliced" cuts the columns into slices whose part of x (<= 2 MiB by
  * default) stays in one XCD's L2; one launch per slice adds into y.
  * "LDS x-tiled" gives each 1024-thread workgroup a block of <= 32768 rows
- * whose partial sums stay in LDS and streams x through LDS in 8128-column
+ * whose partial sums stay in LDS and streams x through LDS in 8064-column
  * slices, a block's slices split into up to 8 column groups (for wide x with
  * scattered columns; needs width >= 4).  The
  * reference's method ids (e.g. texture variants) map onto these. */
