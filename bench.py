@@ -421,8 +421,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the Cocktail composite leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the mixed-precision leg")
-    ap.add_argument("--c4-matrices", type=int, default=256,
-                    help="matrices of the batched leg (c4 names 1024; 0 skips the leg)")
+    ap.add_argument("--c4-matrices", type=int, default=1024,
+                    help="matrices of the batched leg (default 1024, the c4 set; 0 skips the leg)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported "
