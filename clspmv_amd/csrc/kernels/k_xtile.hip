@@ -199,7 +199,7 @@ struct XtArgs {
     const u32x2* idx2;
     const void* val;      // chunk values, XtVal<CODEC>::T per lane
     const float* vscale;  // FP16: row scales; FP8: chunk scales
-    int cols, last_chunk;
+    int cols;
     const float* x;
     float* y;
 };
@@ -217,11 +217,13 @@ __device__ __forceinline__ void xt_fetch_x(XtSet<CODEC>& S, const XtArgs& a, int
 template <int CODEC>
 __device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int t, int wave, int lane) {
     typedef typename XtVal<CODEC>::T V;
-    const int c0 = a.tile_chunk[t];
+    const int c0 = a.tile_chunk[t], c1 = a.tile_chunk[t + 1];
 #pragma unroll
     for (int k = 0; k < kXtRounds; ++k) {
-        // wave-uniform chunk index (the FP8 scale becomes a scalar load)
-        const int c = __builtin_amdgcn_readfirstlane(min(c0 + wave + k * kXtWaves, a.last_chunk));
+        // wave-uniform chunk index (the FP8 scale becomes a scalar load);
+        // a wave with no chunk in this round reloads the tile's last chunk
+        // (a cache hit) rather than fetching the next tile's entries twice
+        const int c = __builtin_amdgcn_readfirstlane(min(c0 + wave + k * kXtWaves, c1 - 1));
         S.e[k] = __builtin_nontemporal_load(a.idx2 + (size_t)c * 64 + lane);
         S.v[k] = __builtin_nontemporal_load(reinterpret_cast<const V*>(a.val) + (size_t)c * 64 + lane);
         if constexpr (CODEC == kCodecFp8Slice) S.s[k] = a.vscale[c];
@@ -293,11 +295,11 @@ template <int CODEC>
 __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     const int* __restrict__ blk_row, const int* __restrict__ blk_tile, const int* __restrict__ tile_slice,
     const int* __restrict__ tile_chunk, const u32x2* __restrict__ idx2, const void* __restrict__ val,
-    const float* __restrict__ vscale, int cols, int last_chunk, const float* __restrict__ x, float* __restrict__ y,
+    const float* __restrict__ vscale, int cols, const float* __restrict__ x, float* __restrict__ y,
     float* __restrict__ partial, int groups, int rows) {
     // the metadata is read-only and unaliased, so its wave-uniform reads
     // become scalar loads that do not wait on the vector-load queue
-    const XtArgs a{blk_row, blk_tile, tile_slice, tile_chunk, idx2, val, vscale, cols, last_chunk, x, y};
+    const XtArgs a{blk_row, blk_tile, tile_slice, tile_chunk, idx2, val, vscale, cols, x, y};
     __shared__ float y_l[kXtRows];
     __shared__ float4 x_l4[kXtX4];
     __shared__ int c_row[2][kXtWaves];
@@ -359,7 +361,7 @@ int launch_xtile(const DevXtile& m, const float* x, float* y, hipStream_t st) {
 #define CLSPMV_XTILE(C)                                                                                          \
     hipLaunchKernelGGL(dev::xtile_kernel<C>, dim3(m.nblocks), dim3(kXtThreads), 0, st, m.blk_row, m.blk_tile, \
                        m.tile_slice, m.tile_chunk, reinterpret_cast<const dev::u32x2*>(m.idx), m.val, m.vscale,   \
-                       m.cols, m.nchunks - 1, x, y, m.partial, m.groups, m.rows)
+                       m.cols, x, y, m.partial, m.groups, m.rows)
     switch (m.codec) {
         case kCodecBf16x2: CLSPMV_XTILE(kCodecBf16x2); break;
         case kCodecFp16Row: CLSPMV_XTILE(kCodecFp16Row); break;
