@@ -222,10 +222,22 @@ __global__ __launch_bounds__(256) void bsr_sliced_kernel(const int* __restrict__
     }
 }
 
-__global__ void copy_probe_kernel(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
-    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * 256;
-    for (; i < n4; i += stride) dst[i] = ld_stream(src + i);
+// Plain copy of n4 float4 (the measured streaming rate): four independent
+// 16-byte loads per thread in flight, one workgroup per 1024 float4.
+__global__ __launch_bounds__(256) void copy_probe_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                         size_t n4) {
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    float4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t i = base + (size_t)u * 256;
+        v[u] = i < n4 ? ld_stream(src + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const size_t i = base + (size_t)u * 256;
+        if (i < n4) dst[i] = v[u];
+    }
 }
 
 __global__ void empty_kernel() {}
@@ -332,7 +344,8 @@ int launch_bsr(const DevBsr& m, const float* x, float* y, hipStream_t st) {
 }
 
 int launch_copy_probe(const float4* src, float4* dst, size_t n4, hipStream_t st) {
-    hipLaunchKernelGGL(dev::copy_probe_kernel, dim3(4096), dim3(256), 0, st, src, dst, n4);
+    if (n4 == 0) return 0;
+    hipLaunchKernelGGL(dev::copy_probe_kernel, dim3((unsigned)((n4 + 1023) / 1024)), dim3(256), 0, st, src, dst, n4);
     return (int)hipGetLastError();
 }
 
