@@ -171,7 +171,8 @@ __global__ __launch_bounds__(256) void spmm_kernel(const SpmmMat* __restrict__ m
     }
 }
 
-// One thread per (long row, right-hand side): y += carries in chunk order.
+// One thread per (long row, right-hand side): y += carries in chunk order,
+// summed in FP64 (the cross-workgroup part of a long row's sum).
 __global__ __launch_bounds__(256) void spmm_finalize_kernel(const SpmmMat* __restrict__ mats,
                                                             const int4* __restrict__ longs, int n_long,
                                                             const float* __restrict__ carry) {
@@ -181,9 +182,10 @@ __global__ __launch_bounds__(256) void spmm_finalize_kernel(const SpmmMat* __res
     const int4 d = longs[i];  // {mat, row, first slot, chunks}
     const SpmmMat m = mats[d.x];
     if (j >= m.k) return;
-    float s = 0.f;
-    for (int c = 0; c < d.w; ++c) s += carry[(size_t)(d.z + c) * 16 + j];
-    m.y[(size_t)d.y * m.k + j] += s;
+    double s = 0.0;
+    for (int c = 0; c < d.w; ++c) s += (double)carry[(size_t)(d.z + c) * 16 + j];
+    float* yp = m.y + (size_t)d.y * m.k + j;
+    *yp = (float)((double)*yp + s);
 }
 
 }  // namespace dev

@@ -344,14 +344,15 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     }
 }
 
-// y[r] += partial[0][r] + partial[1][r] + ... in group order.
+// y[r] += partial[0][r] + partial[1][r] + ... in group order, in FP64 (the
+// cross-workgroup part of a row's sum).
 __global__ __launch_bounds__(256) void xtile_reduce_kernel(const float* __restrict__ partial, int groups, int rows,
                                                            float* __restrict__ y) {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= rows) return;
-    float s = partial[r];
-    for (int g = 1; g < groups; ++g) s += partial[(size_t)g * rows + r];
-    y[r] += s;
+    double s = partial[r];
+    for (int g = 1; g < groups; ++g) s += (double)partial[(size_t)g * rows + r];
+    y[r] = (float)((double)y[r] + s);
 }
 
 }  // namespace dev
