@@ -57,6 +57,8 @@ def lib():
             f = getattr(L, name)
             f.restype = None
             f.argtypes = args
+        L.oracle_csr_spmv_omp.restype = C.c_int
+        L.oracle_csr_spmv_omp.argtypes = [C.c_int, _IP, _IP, _FP, _FP, _FP]
         _o = L
     return _o
 
@@ -120,6 +122,19 @@ def abs_row_sum(rows, cols, vals, x, height):
     lib().oracle_abs_row_sum(rows.size, height, _p(rows, C.c_int), _p(cols, C.c_int), _p(vals, C.c_float),
                              _p(x, C.c_float), _p(s, C.c_double))
     return s
+
+
+def csr_spmv_omp(rp, ci, cv, x):
+    """Project CPU baseline (not the reference): FP32 CSR SpMV on all host
+    cores with OpenMP.  Returns (y, threads used)."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    cv = np.ascontiguousarray(cv, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty(rp.size - 1, np.float32)
+    n = lib().oracle_csr_spmv_omp(rp.size - 1, _p(rp, C.c_int), _p(ci, C.c_int), _p(cv, C.c_float),
+                                  _p(x, C.c_float), _p(y, C.c_float))
+    return y, n
 
 
 def two_vec_compare(a, b):

@@ -21,6 +21,9 @@
  *                         formulas of matrix_storage.h:1872-1895, :2150-2196;
  *                         used to check each layout's indexing.
  *   oracle_two_vec_compare  the reference's report (A/src/util.cpp:93-131).
+ *   oracle_csr_spmv_omp   y = A x over CSR rows with OpenMP on all host
+ *                         cores: a project CPU baseline (not the reference),
+ *                         timed beside the reference's one-core SpMV.
  *
  * How each function is pinned (tests/test_io_oracle_abi.py): closed forms (5-point
  * Laplacian with x = 1 gives 0 / 1 / 2 exactly; a stencil with unit values
@@ -29,6 +32,7 @@
  * (oracle/_ref) on seeded matrices, and the golden vectors in tests/golden.
  */
 #include <math.h>
+#include <omp.h>
 #include <stddef.h>
 #include <string.h>
 
@@ -229,4 +233,22 @@ void oracle_two_vec_compare(const float* a, const float* b, int n, double* out) 
     out[0] = sqrt(sum);
     out[1] = maxdiff;
     out[2] = maxratio;
+}
+
+/* y = A x, CSR rows split over OpenMP threads (dynamic chunks of 1024 rows);
+ * each row summed in order in FP32.  Returns the number of threads used. */
+int oracle_csr_spmv_omp(int height, const int* rp, const int* col, const float* data, const float* x, float* y) {
+    int used = 1;
+#pragma omp parallel
+    {
+#pragma omp single
+        used = omp_get_num_threads();
+#pragma omp for schedule(dynamic, 1024)
+        for (int r = 0; r < height; ++r) {
+            float s = 0.0f;
+            for (int k = rp[r]; k < rp[r + 1]; ++k) s += data[k] * x[col[k]];
+            y[r] = s;
+        }
+    }
+    return used;
 }

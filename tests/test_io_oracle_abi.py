@@ -141,6 +141,24 @@ def test_per_format_oracles_match_dense():
     assert np.allclose(y, ref, atol=1e-4)
 
 
+def test_project_omp_csr_matches_serial_and_f64():
+    # the project CPU baseline (OpenMP over rows): each row summed in order,
+    # so bitwise equal to the serial CSR oracle; within FP32 of the FP64 product
+    coo = M.gen_zipf_rows(20000, 1.8, 16.0, 3000, 3)
+    csr = M.coo2csr(coo)
+    x = M.gen_vector(coo.width, 1)
+    y, threads = O.csr_spmv_omp(csr.row_ptr, csr.col_id, csr.data, x)
+    assert threads >= 1
+    ys = np.zeros(coo.height, np.float32)
+    O.lib().oracle_csr_spmv(coo.height, O._p(np.ascontiguousarray(csr.row_ptr), C.c_int),
+                            O._p(np.ascontiguousarray(csr.col_id), C.c_int),
+                            O._p(np.ascontiguousarray(csr.data), C.c_float), O._p(x, C.c_float), O._p(ys, C.c_float))
+    assert np.array_equal(y, ys)
+    y64 = O.spmv_f64(coo.row_id, coo.col_id, coo.data, x, coo.height)
+    S = O.abs_row_sum(coo.row_id, coo.col_id, coo.data, x, coo.height)
+    assert np.max(np.abs(y - y64) / np.maximum(S, 1e-30)) < 1e-5
+
+
 def test_two_vec_compare_is_sign_blind():
     a = np.array([1.0, -2.0, 3.0], np.float32)
     b = np.array([1.0, 2.0, 3.5], np.float32)
