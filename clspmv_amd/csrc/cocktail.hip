@@ -28,6 +28,7 @@ struct clspmv_gpumat_s {
     DevBuf x, y;
     int width = 0, height = 0;
     std::vector<clspmv_part_t> parts;  // in do_spmv order
+    std::vector<int> row0;             // first row of each part (row-range parts of the flat rows)
     ~clspmv_gpumat_s() {
         for (auto p : parts) clspmv_part_free(p);
         x.release();
@@ -109,6 +110,7 @@ int add_part(clspmv_gpumat_s* g, int which, const clspmv_cocktail* m) {
     }
     if (rc) return rc;
     g->parts.push_back(p);
+    g->row0.push_back(0);
     return CLSPMV_OK;
 }
 
@@ -122,7 +124,81 @@ int add_part(clspmv_gpumat_s* g, int which, const clspmv_cocktail* m) {
 // clspmv_set_option("cocktail_merge_flat", 0).
 bool g_merge_flat = true;
 
-int merge_flat(clspmv_gpumat_s* g, const clspmv_cocktail* m) {
+// The merged flat rows are then cut into runs of row blocks (32768 rows)
+// by where their columns lie: a block is "local" when at least 3/4 of its
+// entries are within 65536 columns of the diagonal (their x stays in L2:
+// CSR adaptive row blocks gather it cheaply), else "scattered" (the
+// x-tiled or column-sliced layout, by the cost rule above, on the run's
+// own shape).  Each run is one CSR part over its rows, launched with y
+// offset to its first row.  c2's band and block rows are local, its Zipf
+// rows scattered.  Switched off with
+// clspmv_set_option("cocktail_split_local", 0).
+bool g_split_local = true;
+constexpr int kRunBlock = 32768;
+constexpr long long kLocalWindow = 65536;
+
+int add_flat_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& m, int meth, int row0) {
+    clspmv_part_t p = nullptr;
+    int rc = clspmv_part_csr_vector(g->ctx, &m, meth, &p);
+    if (rc) return rc;
+    g->parts.push_back(p);
+    g->row0.push_back(row0);
+    return CLSPMV_OK;
+}
+
+// Returns kOneRun (nothing added) when single_ok is false and the rows form
+// a single run: the caller then keeps the partition's own flat part.
+constexpr int kOneRun = 1;
+
+int add_flat_rows(clspmv_gpumat_s* g, const clspmv_csr_matrix& merged, const std::vector<int>& rp, bool single_ok) {
+    const int rows = merged.matinfo.height, cols = merged.matinfo.width;
+    auto scattered_meth = [&](long long r, long long n) {
+        return !wide_x(cols) ? 1 : xtile_pays(r, cols, n) ? kXtile : 2;
+    };
+    const int nbk = (rows + kRunBlock - 1) / kRunBlock;
+    std::vector<signed char> local((size_t)std::max(1, nbk), 1);
+    if (g_split_local) {
+        for (int b = 0; b < nbk; ++b) {
+            const int r0 = b * kRunBlock, r1 = std::min(rows, r0 + kRunBlock);
+            long long n = 0, near = 0;
+            for (int r = r0; r < r1; ++r)
+                for (int k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) {
+                    const long long d = (long long)merged.csr_col_id[k] - r;
+                    near += (d <= kLocalWindow && d >= -kLocalWindow);
+                    ++n;
+                }
+            local[(size_t)b] = n == 0 ? (b ? local[(size_t)b - 1] : 1) : (4 * near >= 3 * n);
+        }
+    }
+    // runs of equal class; one run = the whole matrix as before
+    std::vector<int> cut{0};
+    for (int b = 1; b < nbk; ++b)
+        if (local[(size_t)b] != local[(size_t)b - 1]) cut.push_back(b * kRunBlock);
+    cut.push_back(rows);
+    if (cut.size() == 2) {
+        if (!single_ok) return kOneRun;
+        const long long n = rp[(size_t)rows];
+        return add_flat_part(g, merged, local[0] && g_split_local && nbk > 0 ? 1 : scattered_meth(rows, n), 0);
+    }
+    for (size_t i = 0; i + 1 < cut.size(); ++i) {
+        const int r0 = cut[i], r1 = cut[i + 1];
+        std::vector<int> sub_rp((size_t)(r1 - r0) + 1);
+        for (int r = r0; r <= r1; ++r) sub_rp[(size_t)(r - r0)] = rp[(size_t)r] - rp[(size_t)r0];
+        clspmv_csr_matrix sub{};
+        sub.matinfo.width = cols;
+        sub.matinfo.height = r1 - r0;
+        sub.matinfo.nnz = sub_rp.back();
+        sub.csr_row_ptr = sub_rp.data();
+        sub.csr_col_id = merged.csr_col_id + rp[(size_t)r0];
+        sub.csr_data = merged.csr_data + rp[(size_t)r0];
+        const bool loc = local[(size_t)(r0 / kRunBlock)];
+        const int rc = add_flat_part(g, sub, loc ? 1 : scattered_meth(r1 - r0, sub_rp.back()), r0);
+        if (rc) return rc;
+    }
+    return CLSPMV_OK;
+}
+
+int merge_flat(clspmv_gpumat_s* g, const clspmv_cocktail* m, bool single_ok) {
     const int rows = m->mat_height, cols = m->mat_width;
     std::vector<long long> cnt((size_t)rows + 1, 0);
     const clspmv_ell_matrix& e = m->ell;
@@ -192,13 +268,7 @@ int merge_flat(clspmv_gpumat_s* g, const clspmv_cocktail* m) {
     merged.csr_row_ptr = rp.data();
     merged.csr_col_id = ci.data();
     merged.csr_data = cv.data();
-    const long long stored = (long long)ci.size();
-    const int meth = !wide_x(cols) ? 1 : xtile_pays(rows, cols, stored) ? kXtile : 2;
-    clspmv_part_t p = nullptr;
-    int rc = clspmv_part_csr_vector(g->ctx, &merged, meth, &p);
-    if (rc) return rc;
-    g->parts.push_back(p);
-    return CLSPMV_OK;
+    return add_flat_rows(g, merged, rp, single_ok);
 }
 
 int check_size(clspmv_gpumat_t g, int n, int limit) {
@@ -233,10 +303,19 @@ int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* m, const fl
     const unsigned char use[9] = {m->ifusebdia, m->ifusedia, m->ifusesbell, m->ifusebell, m->ifusebcsr,
                                   m->ifusesell, m->ifuseell, m->ifusecsr, m->ifusecoo};
     const int nflat = use[5] + use[6] + use[7] + use[8];
-    const bool merge = g_merge_flat && nflat >= 2;
+    // a single flat part goes through the merge too when it may split into
+    // row runs; if it does not, it keeps its own format
+    const bool merge = g_merge_flat && (nflat >= 2 || (nflat == 1 && g_split_local));
     for (int i = 0; i < 9 && !rc; ++i)
         if (use[i] && !(merge && i >= 5)) rc = add_part(g, i, m);
-    if (!rc && merge) rc = merge_flat(g, m);
+    if (!rc && merge) {
+        rc = merge_flat(g, m, nflat >= 2);
+        if (rc == kOneRun) {
+            rc = CLSPMV_OK;
+            for (int i = 5; i < 9 && !rc; ++i)
+                if (use[i]) rc = add_part(g, i, m);
+        }
+    }
     if (!rc) rc = clspmv_synchronize(ctx);
     if (rc) {
         delete g;
@@ -260,8 +339,8 @@ int clspmv_do_spmv(clspmv_gpumat_t g, int ntimes) {
     const float* x = (const float*)g->x.ptr;
     float* y = (float*)g->y.ptr;
     for (int i = 0; i < ntimes; ++i)
-        for (auto p : g->parts) {
-            int e = clspmv_part_launch(p, x, y, st);
+        for (size_t j = 0; j < g->parts.size(); ++j) {
+            int e = clspmv_part_launch(g->parts[j], x, y + g->row0[j], st);
             if (e) return hip_status((hipError_t)e, "do_spmv launch");
         }
     return CLSPMV_OK;
@@ -304,6 +383,10 @@ int clspmv_gpumat_num_parts(clspmv_gpumat_t g) { return g ? (int)g->parts.size()
 __attribute__((visibility("hidden"))) int clspmv_cocktail_option(const char* name, long long value) {
     if (name && std::strcmp(name, "cocktail_merge_flat") == 0) {
         g_merge_flat = value != 0;
+        return CLSPMV_OK;
+    }
+    if (name && std::strcmp(name, "cocktail_split_local") == 0) {
+        g_split_local = value != 0;
         return CLSPMV_OK;
     }
     return CLSPMV_E_INVALID;

@@ -360,7 +360,12 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  *                       partial-sum pass adds the groups' sums to y);
  *   "cocktail_merge_flat"  1 (default): clspmv_init_mat_kernels runs the
  *                       flat parts (ELL, SELL, CSR, COO) of a partition as
- *                       one merged row-list part; 0: one part each. */
+ *                       one merged row-list part; 0: one part each;
+ *   "cocktail_split_local" 1 (default): the merged flat rows are cut into
+ *                       runs of 32768-row blocks whose columns lie near the
+ *                       diagonal (CSR row blocks) or not (x-tiled or
+ *                       column-sliced layout), one part per run; 0: one
+ *                       part. */
 int clspmv_set_option(const char* name, long long value);
 int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info);
 int clspmv_part_free(clspmv_part_t p);

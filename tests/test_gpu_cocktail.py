@@ -25,9 +25,12 @@ def tables():
 
 @pytest.fixture(params=[1, 0], ids=["merged_flat", "separate_flat"])
 def merge_flat(request):
+    # part counts below assume one merged flat part (no split into row runs)
     L.check(L.lib().clspmv_set_option(b"cocktail_merge_flat", request.param), "set_option")
+    L.check(L.lib().clspmv_set_option(b"cocktail_split_local", 0), "set_option")
     yield request.param
     L.check(L.lib().clspmv_set_option(b"cocktail_merge_flat", 1), "set_option")
+    L.check(L.lib().clspmv_set_option(b"cocktail_split_local", 1), "set_option")
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -59,6 +62,51 @@ def test_cocktail_mi355x_tables(ctx, merge_flat, case):
     rep = O.parity_report(gpu.result_to_cpu(), coo, x, y0)
     assert rep["ok"], (case, mat.parts(), rep)
     gpu.free()
+
+
+def _local_scattered(width, seed):
+    """3 x 32768 rows, 8 entries each: the first and last 32768 rows within
+    2000 columns of the diagonal, the middle ones uniform over the width."""
+    rng = np.random.default_rng(seed)
+    B = 32768
+    rows = np.repeat(np.arange(3 * B, dtype=np.int64), 8)
+    near = np.clip(rows + rng.integers(-2000, 2001, rows.size), 0, width - 1)
+    far = rng.integers(0, width, rows.size)
+    cols = np.where((rows >= B) & (rows < 2 * B), far, near)
+    key = np.unique(rows * width + cols)
+    r, c = (key // width).astype(np.int32), (key % width).astype(np.int32)
+    v = rng.standard_normal(r.size).astype(np.float32)
+    return M.CooMatrix.from_arrays(r, c, v, width, 3 * B)
+
+
+@pytest.mark.parametrize("width", [1 << 20, 1 << 22], ids=["x_in_l2", "wide_x"])
+def test_cocktail_split_local(ctx, tables, width):
+    # the merged flat rows become three row-run parts (local, scattered,
+    # local) launched with y offset to their first rows; same product as one
+    # part
+    coo = _local_scattered(width, 31)
+    mat = CK.analyze_matrix(coo.copy(), tables)
+    flat = [n for n in mat.parts() if n in ("sell", "ell", "csr", "coo")]
+    assert flat
+    other = len(mat.parts()) - len(flat)
+    x = M.gen_vector(coo.width, 25)
+    y0 = M.gen_vector(coo.height, 26)
+    ys = {}
+    for split in (1, 0):
+        L.check(L.lib().clspmv_set_option(b"cocktail_split_local", split), "set_option")
+        try:
+            gpu = CK.GpuCocktail(ctx, mat, x, y0)
+        finally:
+            L.check(L.lib().clspmv_set_option(b"cocktail_split_local", 1), "set_option")
+        parts = gpu.part_infos()
+        assert len(parts) == other + (3 if split else 1), (split, mat.parts(), parts)
+        if split:
+            assert sorted(p["rows"] for p in parts[other:]) == [32768, 32768, 32768]
+        gpu.do_spmv(1)
+        ys[split] = gpu.result_to_cpu()
+        rep = O.parity_report(ys[split], coo, x, y0)
+        assert rep["ok"], (split, width, rep)
+        gpu.free()
 
 
 def test_power_iteration(ctx, tables):
