@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void bell_kernel(const int* __restrict__ col, 
                                                    int Hc, int F4q, int slots, int brows,
                                                    const float* __restrict__ x, float* __restrict__ y, int rows,
                                                    int cols) {
-    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int r = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (r >= brows) return;
     float acc[BH];
 #pragma unroll
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void sbell_kernel(const int* __restrict__ slic
                                                     const float4* __restrict__ data, int S, int brows,
                                                     const float* __restrict__ x, float* __restrict__ y, int rows,
                                                     int cols) {
-    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int r = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (r >= brows) return;
     const int s = r / S, t = r - s * S;
     const int b = slice_ptr[s];
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void bcsr_kernel(const int* __restrict__ rp, c
                                                    const float4* __restrict__ data, int N4, int brows,
                                                    const float* __restrict__ x, float* __restrict__ y, int rows,
                                                    int cols) {
-    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int gid = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const int lane = threadIdx.x & (L - 1);
     const int r = gid / L;
     if (r >= brows) return;
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void bsr_kernel(const int* __restrict__ rp, co
                                                   const float* __restrict__ data, int stride, int brows,
                                                   const float* __restrict__ x, float* __restrict__ y, int rows,
                                                   int cols) {
-    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int gid = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const int lane = threadIdx.x & (L - 1);
     const int r = gid / L;
     if (r >= brows) return;
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void bsr_sliced_kernel(const int* __restrict__
                                                          const float* __restrict__ x, float* __restrict__ y,
                                                          int rows, int cols) {
     constexpr int G = bsr_group(BH * BW);
-    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int r = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (r >= brows) return;
     const int s = r >> 6, t = r & 63;
     const int b = sp[s];

@@ -19,7 +19,7 @@ namespace dev {
 __global__ __launch_bounds__(256) void ell_row_kernel(const int* __restrict__ col, const float* __restrict__ val, int H,
                                                       int ell_num, const float* __restrict__ x,
                                                       float* __restrict__ y, int rows) {
-    const int row = blockIdx.x * 256 + threadIdx.x;
+    const int row = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (row >= rows) return;
     float acc = y[row];
     const int* cp = col + row;
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void ell_row_kernel(const int* __restrict__ co
 __global__ __launch_bounds__(256) void ell_row4_kernel(const int4* __restrict__ col, const float4* __restrict__ val,
                                                        int H4, int ell_num, const float* __restrict__ x,
                                                        float* __restrict__ y, int rows) {
-    const int r4 = blockIdx.x * 256 + threadIdx.x;
+    const int r4 = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const int row = r4 * 4;
     if (row >= rows) return;
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void sell_row_kernel(const int* __restrict__ s
                                                        const float* __restrict__ val, int S,
                                                        const float* __restrict__ x, float* __restrict__ y,
                                                        int rows) {
-    const int row = blockIdx.x * 256 + threadIdx.x;
+    const int row = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (row >= rows) return;
     const int s = row / S;
     const int b = slice_ptr[s];

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void bsr_mixed_kernel(const int* __restrict__ 
                                                         int nsl, const float* __restrict__ x, float* __restrict__ y,
                                                         int rows, int cols) {
     constexpr int G = bsr_group(BH * BW);
-    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int r = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (r >= brows) return;
     // a wave's 64 block rows are one slice: wave-uniform slice quantities
     const int s = __builtin_amdgcn_readfirstlane(r >> 6), t = r & 63;
