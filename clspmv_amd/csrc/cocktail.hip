@@ -126,12 +126,12 @@ bool g_merge_flat = true;
 
 // The merged flat rows are then cut into runs of row blocks (32768 rows)
 // by where their columns lie: a block is "local" when at least 3/4 of its
-// entries are within 65536 columns of the diagonal (their x stays in L2:
-// CSR adaptive row blocks gather it cheaply), else "scattered" (the
-// x-tiled or column-sliced layout, by the cost rule above, on the run's
-// own shape).  Each run is one CSR part over its rows, launched with y
-// offset to its first row.  c2's band and block rows are local, its Zipf
-// rows scattered.  Switched off with
+// entries are within 65536 columns of the diagonal (their x stays in L2),
+// else "scattered" (the x-tiled or column-sliced layout, by the cost rule
+// above, on the run's own shape); local blocks are refined by shape (see
+// RunClass).  Each run is one part over its rows, launched with y offset to
+// its first row.  c2's band rows run as SELL, its 4 x 4 block rows as
+// sliced BSR, its Zipf rows x-tiled.  Switched off with
 // clspmv_set_option("cocktail_split_local", 0).
 bool g_split_local = true;
 constexpr int kRunBlock = 32768;
@@ -150,36 +150,110 @@ int add_flat_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& m, int meth, int 
 // a single run: the caller then keeps the partition's own flat part.
 constexpr int kOneRun = 1;
 
-int add_flat_rows(clspmv_gpumat_s* g, const clspmv_csr_matrix& merged, const std::vector<int>& rp, bool single_ok) {
-    const int rows = merged.matinfo.height, cols = merged.matinfo.width;
-    auto scattered_meth = [&](long long r, long long n) {
-        return !wide_x(cols) ? 1 : xtile_pays(r, cols, n) ? kXtile : 2;
-    };
-    const int nbk = (rows + kRunBlock - 1) / kRunBlock;
-    std::vector<signed char> local((size_t)std::max(1, nbk), 1);
-    if (g_split_local) {
-        for (int b = 0; b < nbk; ++b) {
-            const int r0 = b * kRunBlock, r1 = std::min(rows, r0 + kRunBlock);
-            long long n = 0, near = 0;
-            for (int r = r0; r < r1; ++r)
-                for (int k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) {
-                    const long long d = (long long)merged.csr_col_id[k] - r;
-                    near += (d <= kLocalWindow && d >= -kLocalWindow);
-                    ++n;
-                }
-            local[(size_t)b] = n == 0 ? (b ? local[(size_t)b - 1] : 1) : (4 * near >= 3 * n);
+// Row-run classes.  Local runs are refined by shape: rows of dense aligned
+// b x b blocks (b = 4 or 2; >= 95 % of the entries in full blocks) run as
+// native sliced BSR, rows of near-equal length (SELL with 32-row slices
+// pads <= 25 %) as SELL, other rows as CSR row blocks.
+enum RunClass : signed char { kScattered = 0, kLocalCsr = 1, kLocalSell = 2, kLocalBsr2 = 12, kLocalBsr4 = 14 };
+
+// Share of a row block's entries in full b x b blocks (block rows counted
+// from r0, which is a multiple of b).
+double full_block_share(const clspmv_csr_matrix& m, const std::vector<int>& rp, int r0, int r1, int b) {
+    long long n = 0, full = 0;
+    std::vector<int> keys;
+    for (int br = r0; br < r1; br += b) {
+        keys.clear();
+        for (int r = br; r < std::min(r1, br + b); ++r)
+            for (int k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) keys.push_back(m.csr_col_id[k] / b);
+        std::sort(keys.begin(), keys.end());
+        n += (long long)keys.size();
+        for (size_t i = 0; i < keys.size();) {
+            size_t j = i;
+            while (j < keys.size() && keys[j] == keys[i]) ++j;
+            if ((long long)(j - i) == (long long)b * b) full += (long long)(j - i);
+            i = j;
         }
     }
-    // runs of equal class; one run = the whole matrix as before
+    return n ? (double)full / (double)n : 0.0;
+}
+
+signed char classify_block(const clspmv_csr_matrix& m, const std::vector<int>& rp, int r0, int r1) {
+    long long n = 0, near = 0;
+    for (int r = r0; r < r1; ++r)
+        for (int k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) {
+            const long long d = (long long)m.csr_col_id[k] - r;
+            near += (d <= kLocalWindow && d >= -kLocalWindow);
+            ++n;
+        }
+    if (n == 0) return -1;
+    if (4 * near < 3 * n) return kScattered;
+    if (full_block_share(m, rp, r0, r1, 4) >= 0.95) return kLocalBsr4;
+    if (full_block_share(m, rp, r0, r1, 2) >= 0.95) return kLocalBsr2;
+    long long slots = 0;
+    for (int s = r0; s < r1; s += 32) {
+        int w = 0;
+        for (int r = s; r < std::min(r1, s + 32); ++r) w = std::max(w, rp[(size_t)r + 1] - rp[(size_t)r]);
+        slots += 32LL * w;
+    }
+    return 4 * slots <= 5 * n ? kLocalSell : kLocalCsr;
+}
+
+// One run as a part of its class; rows [0, sub.height) of `sub`.
+int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char cls, int row0) {
+    const int cols = sub.matinfo.width, rows = sub.matinfo.height;
+    const long long n = sub.matinfo.nnz;
+    if (cls == kScattered) return add_flat_part(g, sub, !wide_x(cols) ? 1 : xtile_pays(rows, cols, n) ? kXtile : 2, row0);
+    if (cls == kLocalCsr) return add_flat_part(g, sub, 1, row0);
+    // SELL / BSR: through the host conversions of a COO copy of the run
+    clspmv_coo_matrix coo{};
+    int rc = clspmv_alloc_coo(&coo, cols, rows, (int)n);
+    if (rc) return rc;
+    for (int r = 0; r < rows; ++r)
+        for (int k = sub.csr_row_ptr[r]; k < sub.csr_row_ptr[r + 1]; ++k) {
+            coo.coo_row_id[k] = r;
+            coo.coo_col_id[k] = sub.csr_col_id[k];
+            coo.coo_data[k] = sub.csr_data[k];
+        }
+    clspmv_part_t p = nullptr;
+    if (cls == kLocalSell) {
+        clspmv_sell_matrix sell{};
+        rc = clspmv_coo2sell(&coo, &sell, 32);
+        if (!rc) rc = clspmv_part_sell(g->ctx, &sell, 0, &p);
+        clspmv_free_sell(&sell);
+    } else {
+        const unsigned b = cls == kLocalBsr4 ? 4 : 2;
+        clspmv_bsr_matrix bsr{};
+        rc = clspmv_coo2bsr(&coo, &bsr, b, b);
+        if (!rc) rc = clspmv_part_bsr(g->ctx, &bsr, 0, &p);
+        clspmv_free_bsr(&bsr);
+    }
+    clspmv_free_coo(&coo);
+    if (rc) return rc;
+    g->parts.push_back(p);
+    g->row0.push_back(row0);
+    return CLSPMV_OK;
+}
+
+int add_flat_rows(clspmv_gpumat_s* g, const clspmv_csr_matrix& merged, const std::vector<int>& rp, bool single_ok) {
+    const int rows = merged.matinfo.height, cols = merged.matinfo.width;
+    const int nbk = (rows + kRunBlock - 1) / kRunBlock;
+    if (!g_split_local || nbk == 0) {
+        if (!single_ok) return kOneRun;
+        const long long n = rows ? rp[(size_t)rows] : 0;
+        return add_flat_part(g, merged, !wide_x(cols) ? 1 : xtile_pays(rows, cols, n) ? kXtile : 2, 0);
+    }
+    std::vector<signed char> cls((size_t)nbk);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(omp_threads())
+    for (int b = 0; b < nbk; ++b)
+        cls[(size_t)b] = classify_block(merged, rp, b * kRunBlock, std::min(rows, (b + 1) * kRunBlock));
+    for (int b = 0; b < nbk; ++b)  // empty blocks join the previous run
+        if (cls[(size_t)b] < 0) cls[(size_t)b] = b ? cls[(size_t)b - 1] : kLocalCsr;
+    // runs of equal class
     std::vector<int> cut{0};
     for (int b = 1; b < nbk; ++b)
-        if (local[(size_t)b] != local[(size_t)b - 1]) cut.push_back(b * kRunBlock);
+        if (cls[(size_t)b] != cls[(size_t)b - 1]) cut.push_back(b * kRunBlock);
     cut.push_back(rows);
-    if (cut.size() == 2) {
-        if (!single_ok) return kOneRun;
-        const long long n = rp[(size_t)rows];
-        return add_flat_part(g, merged, local[0] && g_split_local && nbk > 0 ? 1 : scattered_meth(rows, n), 0);
-    }
+    if (cut.size() == 2 && !single_ok && cls[0] != kLocalSell && cls[0] < kLocalBsr2) return kOneRun;
     for (size_t i = 0; i + 1 < cut.size(); ++i) {
         const int r0 = cut[i], r1 = cut[i + 1];
         std::vector<int> sub_rp((size_t)(r1 - r0) + 1);
@@ -191,8 +265,7 @@ int add_flat_rows(clspmv_gpumat_s* g, const clspmv_csr_matrix& merged, const std
         sub.csr_row_ptr = sub_rp.data();
         sub.csr_col_id = merged.csr_col_id + rp[(size_t)r0];
         sub.csr_data = merged.csr_data + rp[(size_t)r0];
-        const bool loc = local[(size_t)(r0 / kRunBlock)];
-        const int rc = add_flat_part(g, sub, loc ? 1 : scattered_meth(r1 - r0, sub_rp.back()), r0);
+        const int rc = add_run_part(g, sub, cls[(size_t)(r0 / kRunBlock)], r0);
         if (rc) return rc;
     }
     return CLSPMV_OK;

@@ -109,6 +109,46 @@ def test_cocktail_split_local(ctx, tables, width):
         gpu.free()
 
 
+def _blocks_scattered(seed):
+    """2 x 32768 rows of width 2^22: dense aligned 4 x 4 blocks (6 per block
+    row, near the diagonal), then 10 uniform columns per row."""
+    rng = np.random.default_rng(seed)
+    B, W = 32768, 1 << 22
+    br = np.arange(B // 4)
+    bc = np.clip(br[:, None] + rng.integers(-1000, 1001, (B // 4, 6)), 0, W // 4 - 1)
+    rows = [(br[:, None] * 4 + i).repeat(6, 1).ravel() for i in range(4) for _ in range(4)]
+    cols = [(bc * 4 + j).ravel() for _ in range(4) for j in range(4)]
+    r = np.concatenate(rows + [np.repeat(np.arange(B, 2 * B), 10)]).astype(np.int64)
+    c = np.concatenate(cols + [rng.integers(0, W, 10 * B)]).astype(np.int64)
+    key = np.unique(r * W + c)
+    r, c = (key // W).astype(np.int32), (key % W).astype(np.int32)
+    return M.CooMatrix.from_arrays(r, c, rng.standard_normal(r.size).astype(np.float32), W, 2 * B)
+
+
+def test_cocktail_runs_by_shape(ctx, tables):
+    # local rows of dense 4 x 4 blocks run as sliced BSR, local rows of equal
+    # length as SELL, scattered rows x-tiled or column-sliced
+    cases = [(_blocks_scattered(41), ["bsr44_sliced", "scattered"]),
+             (_local_scattered(1 << 22, 43), ["sell", "scattered", "sell"])]
+    for coo, kinds in cases:
+        mat = CK.analyze_matrix(coo.copy(), tables)
+        assert mat.parts() == ["ell"], mat.parts()
+        x = M.gen_vector(coo.width, 27)
+        y0 = M.gen_vector(coo.height, 28)
+        gpu = CK.GpuCocktail(ctx, mat, x, y0)
+        names = [p["kernel"] for p in gpu.part_infos()]
+        assert len(names) == len(kinds), names
+        for name, kind in zip(names, kinds):
+            if kind == "scattered":
+                assert name.startswith(("xtile", "csr_sliced")), names
+            else:
+                assert name.startswith(kind), names
+        gpu.do_spmv(1)
+        rep = O.parity_report(gpu.result_to_cpu(), coo, x, y0)
+        assert rep["ok"], (names, rep)
+        gpu.free()
+
+
 def test_power_iteration(ctx, tables):
     # y = A^k x pattern of the reference's example driver: each do_spmv
     # accumulates into y, then y is copied into x on the device, so
