@@ -221,7 +221,7 @@ __device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int
 #pragma unroll
     for (int k = 0; k < kXtRounds; ++k) {
         // wave-uniform chunk index (the FP8 scale becomes a scalar load);
-        // a wave with no chunk in this round reloads the tile's last chunk
+        // a wave with no chunk in this chunk round reloads the tile's last chunk
         // (a cache hit) rather than fetching the next tile's entries twice
         const int c = __builtin_amdgcn_readfirstlane(min(c0 + wave + k * kXtWaves, c1 - 1));
         S.e[k] = __builtin_nontemporal_load(a.idx2 + (size_t)c * 64 + lane);

@@ -6,7 +6,7 @@ library on a matrix and returns the times; the selector is a small network
 per candidate), trained on the GPU by the library's own kernels on
 autotuner-timed matrices, that predicts the fastest candidate from the
 features alone.  Exhaustive autotuning stays available as its fallback and
-as the judge of its regret (time of the predicted candidate / time of the
+as the measure of its regret (time of the predicted candidate / time of the
 best one - 1).
 
     times = autotune(ctx, coo)                     # one row of timings
