@@ -187,8 +187,17 @@ signed char classify_block(const clspmv_csr_matrix& m, const std::vector<int>& r
         }
     if (n == 0) return -1;
     if (4 * near < 3 * n) return kScattered;
-    if (full_block_share(m, rp, r0, r1, 4) >= 0.95) return kLocalBsr4;
-    if (full_block_share(m, rp, r0, r1, 2) >= 0.95) return kLocalBsr2;
+    // repeated (row, col) entries (a matrix file may hold them) are summed
+    // by the row-list layouts but would collide in a block: no BSR then
+    bool repeats = false;
+    for (int r = r0; r < r1 && !repeats; ++r)
+        for (int k = rp[(size_t)r] + 1; k < rp[(size_t)r + 1]; ++k)
+            if (m.csr_col_id[k] == m.csr_col_id[k - 1]) {
+                repeats = true;
+                break;
+            }
+    if (!repeats && full_block_share(m, rp, r0, r1, 4) >= 0.95) return kLocalBsr4;
+    if (!repeats && full_block_share(m, rp, r0, r1, 2) >= 0.95) return kLocalBsr2;
     long long slots = 0;
     for (int s = r0; s < r1; s += 32) {
         int w = 0;
