@@ -262,7 +262,16 @@ int add_flat_rows(clspmv_gpumat_s* g, const clspmv_csr_matrix& merged, const std
     for (int b = 1; b < nbk; ++b)
         if (cls[(size_t)b] != cls[(size_t)b - 1]) cut.push_back(b * kRunBlock);
     cut.push_back(rows);
-    if (cut.size() == 2 && !single_ok && cls[0] != kLocalSell && cls[0] < kLocalBsr2) return kOneRun;
+    if (cut.size() == 2) {
+        // one run: only a block-structured one changes layout (to sliced
+        // BSR); otherwise the partition's own flat part (single_ok false)
+        // or the merged rows as before
+        if (cls[0] < kLocalBsr2) {
+            if (!single_ok) return kOneRun;
+            const long long n = rp[(size_t)rows];
+            return add_flat_part(g, merged, !wide_x(cols) ? 1 : xtile_pays(rows, cols, n) ? kXtile : 2, 0);
+        }
+    }
     for (size_t i = 0; i + 1 < cut.size(); ++i) {
         const int r0 = cut[i], r1 = cut[i + 1];
         std::vector<int> sub_rp((size_t)(r1 - r0) + 1);
