@@ -306,7 +306,10 @@ def batched_c4(ctx, args):
                 blocks (one launch each);
       selector  the same with the format the learned selector picks;
       batched   one replay of the captured graph (persistent kernel over all
-                matrices)."""
+                matrices);
+    and, on a seeded eighth of the matrices, the plain FP32 Cocktail path
+    (analyzer + k do_spmv per matrix) against a batch of the same
+    matrices."""
     import numpy as np
 
     from clspmv_amd import batch as B
@@ -419,6 +422,53 @@ def batched_c4(ctx, args):
                        "kernel": "learned selector's format, k launches per matrix"}
     out["batched_vs_plain"] = round(ms_p / ms_b, 3)
     out["selector_vs_plain"] = round(ms_p / ms_s, 3)
+
+    # plain FP32 Cocktail per matrix (analyzer with the MI355X tables, k
+    # do_spmv per matrix) on an eighth of the matrices, against one graph
+    # replay of a batch of the same matrices
+    from clspmv_amd import cocktail as CK
+
+    # a seeded eighth of the set (every kind and every k)
+    sub = sorted(np.random.default_rng(8).choice(len(mats), max(1, len(mats) // 8), replace=False).tolist())
+    tables = CK.BenchTables(CK.MI355X_TABLES)
+    t0 = time.perf_counter()
+    objs = []
+    for i in sub:
+        _, coo, k = mats[i]
+        cm = CK.analyze_matrix(coo.copy(), tables)
+        objs.append((CK.GpuCocktail(ctx, cm, np.ones(coo.width, np.float32), np.zeros(coo.height, np.float32)),
+                     cm, k))
+    t_an = time.perf_counter() - t0
+    for gpu, _, k in objs:
+        gpu.do_spmv(k)
+    ctx.sync()
+    ctx.time_begin()
+    for _ in range(reps):
+        for gpu, _, k in objs:
+            gpu.do_spmv(k)
+    ms_c = ctx.time_end() / reps
+    for gpu, cm, _ in objs:
+        gpu.free()
+        cm.free()
+    bs = B.Batch(ctx, [csrs[i] for i in sub], [ks[i] for i in sub])
+    for j, i in enumerate(sub):
+        bs.set_x(j, np.ones((mats[i][1].width, ks[i]), np.float32))
+    bs.capture()
+    bs.run(1)
+    ctx.sync()
+    ctx.time_begin()
+    bs.run(reps)
+    ms_bs = ctx.time_end() / reps
+    bs.free()
+    nz_sub = sum(mats[i][1].nnz * ks[i] for i in sub)
+    out["cocktail_subset"] = {
+        "matrices": len(sub), "nnz_x_rhs": int(nz_sub), "analysis_s": round(t_an, 1),
+        "cocktail_ms": round(ms_c, 4), "cocktail_eff_gnnz_per_s": round(nz_sub / (ms_c * 1e-3) / 1e9, 1),
+        "batched_ms": round(ms_bs, 4), "batched_eff_gnnz_per_s": round(nz_sub / (ms_bs * 1e-3) / 1e9, 1),
+        "batched_vs_cocktail": round(ms_c / ms_bs, 3),
+        "kernel": "Cocktail partition per matrix (MI355X tables), k do_spmv each"}
+    log(f"c4 Cocktail per matrix ({len(sub)} matrices, analysis {t_an:.1f} s): {ms_c:.2f} ms, batched "
+        f"{ms_bs:.2f} ms ({out['cocktail_subset']['batched_vs_cocktail']}x)")
     for csr in csrs:
         csr.free()
     for _, coo, _ in mats:
