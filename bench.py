@@ -200,8 +200,9 @@ def mixed_c3(ctx, args, work, per):
     """Mixed precision (c3) on the c1 matrices and the c2 composite: each c1
     matrix in its c1 layout with values stored as BF16 hi+lo, FP16 with a
     row scale or FP8 E4M3 with a slice scale (DIA and x-tiled CSR parts; BSR
-    with int16 delta block columns), FP32 accumulation; c2 as one x-tiled
-    CSR part against its FP32 Cocktail partition.  Per matrix and codec:
+    with int16 delta block columns), FP32 accumulation; c2 through its
+    Cocktail partition with the codec on the parts that have one (x-tiled
+    and sliced BSR row runs) against the FP32 partition.  Per matrix and codec:
     kernel time (HIP events), stored bytes, effective nonzeros/s, speedup
     over FP32, and the error of one y = A x against the FP64 product."""
     import numpy as np
@@ -246,38 +247,47 @@ def mixed_c3(ctx, args, work, per):
         out[name] = rec
     if args.no_c2:
         return out
-    # c2: the Cocktail composite; FP32 baseline = the Cocktail object with
-    # the MI355X tables, mixed = the whole matrix as an x-tiled CSR part
+    # c2: the Cocktail composite; FP32 = the Cocktail object with the MI355X
+    # tables; mixed = the same partition with the value codec on its parts
+    # that have a mixed-precision variant (x-tiled and sliced BSR row runs)
+    import ctypes as C
+
     from clspmv_amd import cocktail as CK
     from clspmv_amd import workloads as W
 
     coo = W.make_coo("c2", args.scale)
     x = M.gen_vector(coo.width, 7)
-    mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(CK.MI355X_TABLES))
-    gpu = CK.GpuCocktail(ctx, mat, x, np.zeros(coo.height, np.float32))
-    gpu.do_spmv(2)
-    ctx.sync()
-    ctx.time_begin()
-    gpu.do_spmv(k)
-    ms32 = ctx.time_end() / k
-    gpu.result_from_cpu(np.zeros(coo.height, np.float32))
-    gpu.do_spmv(1)
     y64, srow = _fp64_reference(coo, x)
-    e_max, e_norm = _errors(gpu.result_to_cpu(), y64, srow)
-    stored = sum(p["stored_bytes"] for p in gpu.part_infos())
-    rec = {"fp32": {"kernel": "Cocktail: " + "+".join(p["kernel"] for p in gpu.part_infos()), "ms": round(ms32, 5),
-                    "stored_bytes": int(stored), "eff_gnnz_per_s": round(coo.nnz / (ms32 * 1e-3) / 1e9, 1),
-                    "max_rel_err": e_max, "norm_rel_err": e_norm}}
-    gpu.free()
+    mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(CK.MI355X_TABLES))
+    rec = {}
+    ms32 = None
+    for codec, label in ((0, "fp32"), (L.VAL_BF16X2, "bf16x2"), (L.VAL_FP16_ROW, "fp16row"),
+                         (L.VAL_FP8_SLICE, "fp8slice")):
+        L.check(L.lib().clspmv_set_option(b"cocktail_value_codec", C.c_longlong(codec)), "set_option")
+        try:
+            gpu = CK.GpuCocktail(ctx, mat, x, np.zeros(coo.height, np.float32))
+        finally:
+            L.check(L.lib().clspmv_set_option(b"cocktail_value_codec", C.c_longlong(0)), "set_option")
+        gpu.do_spmv(2)
+        ctx.sync()
+        ctx.time_begin()
+        gpu.do_spmv(k)
+        ms = ctx.time_end() / k
+        gpu.result_from_cpu(np.zeros(coo.height, np.float32))
+        gpu.do_spmv(1)
+        e_max, e_norm = _errors(gpu.result_to_cpu(), y64, srow)
+        infos = gpu.part_infos()
+        ms32 = ms if codec == 0 else ms32
+        rec[label] = {"kernel": "Cocktail: " + "+".join(p["kernel"] for p in infos), "ms": round(ms, 5),
+                      "stored_bytes": int(sum(p["stored_bytes"] for p in infos)),
+                      "eff_gnnz_per_s": round(coo.nnz / (ms * 1e-3) / 1e9, 1),
+                      "max_rel_err": e_max, "norm_rel_err": e_norm}
+        if codec:
+            rec[label]["speedup_vs_fp32"] = round(ms32 / ms, 3)
+            log(f"c3 c2 {rec[label]['kernel']}: {ms * 1e3:.1f} us ({ms32 / ms:.2f}x FP32 Cocktail), "
+                f"max rel err {e_max:.2e}, norm rel err {e_norm:.2e}")
+        gpu.free()
     mat.free()
-    dx = M.DeviceArray.from_numpy(ctx, x)
-    dy = M.DeviceArray.from_numpy(ctx, np.zeros(coo.height, np.float32))
-    csr = M.coo2csr(coo)
-    _mixed_codecs(ctx, M, L, rec, L.FMT_CSR_VECTOR, csr, coo, {"dx": dx, "dy": dy}, ms32, y64, srow, time_part,
-                  run_once, name="c2")
-    csr.free()
-    dx.free()
-    dy.free()
     coo.free()
     out["c2"] = rec
     return out

@@ -134,6 +134,10 @@ bool g_merge_flat = true;
 // sliced BSR, its Zipf rows x-tiled.  Switched off with
 // clspmv_set_option("cocktail_split_local", 0).
 bool g_split_local = true;
+// Value codec of the row-run parts that have a mixed-precision variant (the
+// x-tiled and sliced BSR ones): 0 = FP32 (default) or a CLSPMV_VAL_* code
+// ("cocktail_value_codec").  Other parts stay FP32.
+int g_run_codec = 0;
 constexpr int kRunBlock = 32768;
 constexpr long long kLocalWindow = 65536;
 
@@ -211,7 +215,16 @@ signed char classify_block(const clspmv_csr_matrix& m, const std::vector<int>& r
 int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char cls, int row0) {
     const int cols = sub.matinfo.width, rows = sub.matinfo.height;
     const long long n = sub.matinfo.nnz;
-    if (cls == kScattered) return add_flat_part(g, sub, !wide_x(cols) ? 1 : xtile_pays(rows, cols, n) ? kXtile : 2, row0);
+    if (cls == kScattered) {
+        const int meth = !wide_x(cols) ? 1 : xtile_pays(rows, cols, n) ? kXtile : 2;
+        if (meth != kXtile || !g_run_codec) return add_flat_part(g, sub, meth, row0);
+        clspmv_part_t p = nullptr;
+        const int rc = clspmv_part_csr_mixed(g->ctx, &sub, g_run_codec, &p);
+        if (rc) return rc;
+        g->parts.push_back(p);
+        g->row0.push_back(row0);
+        return CLSPMV_OK;
+    }
     if (cls == kLocalCsr) return add_flat_part(g, sub, 1, row0);
     // SELL / BSR: through the host conversions of a COO copy of the run
     clspmv_coo_matrix coo{};
@@ -233,7 +246,8 @@ int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char c
         const unsigned b = cls == kLocalBsr4 ? 4 : 2;
         clspmv_bsr_matrix bsr{};
         rc = clspmv_coo2bsr(&coo, &bsr, b, b);
-        if (!rc) rc = clspmv_part_bsr(g->ctx, &bsr, 0, &p);
+        if (!rc) rc = g_run_codec ? clspmv_part_bsr_mixed(g->ctx, &bsr, g_run_codec, 1, &p)
+                                  : clspmv_part_bsr(g->ctx, &bsr, 0, &p);
         clspmv_free_bsr(&bsr);
     }
     clspmv_free_coo(&coo);
@@ -478,6 +492,11 @@ __attribute__((visibility("hidden"))) int clspmv_cocktail_option(const char* nam
     }
     if (name && std::strcmp(name, "cocktail_split_local") == 0) {
         g_split_local = value != 0;
+        return CLSPMV_OK;
+    }
+    if (name && std::strcmp(name, "cocktail_value_codec") == 0) {
+        if (value < 0 || value > CLSPMV_VAL_FP8_SLICE) return CLSPMV_E_INVALID;
+        g_run_codec = (int)value;
         return CLSPMV_OK;
     }
     return CLSPMV_E_INVALID;

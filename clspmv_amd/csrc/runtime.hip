@@ -1392,7 +1392,8 @@ extern "C" int clspmv_cocktail_option(const char* name, long long value);  // co
 
 int clspmv_set_option(const char* name, long long value) {
     if (!name) return CLSPMV_E_INVALID;
-    if (std::strcmp(name, "cocktail_merge_flat") == 0 || std::strcmp(name, "cocktail_split_local") == 0)
+    if (std::strcmp(name, "cocktail_merge_flat") == 0 || std::strcmp(name, "cocktail_split_local") == 0 ||
+        std::strcmp(name, "cocktail_value_codec") == 0)
         return clspmv_cocktail_option(name, value);
     if (std::strcmp(name, "xtile_max_groups") == 0) {
         if (value < 1 || value > kXtMaxGroups) return CLSPMV_E_INVALID;

@@ -365,7 +365,10 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  *                       runs of 32768-row blocks whose columns lie near the
  *                       diagonal (CSR row blocks) or not (x-tiled or
  *                       column-sliced layout), one part per run; 0: one
- *                       part. */
+ *                       part;
+ *   "cocktail_value_codec" 0 (default) or a CLSPMV_VAL_* codec for the row-run
+ *                       parts that have a mixed-precision variant (x-tiled,
+ *                       sliced BSR); the others stay FP32. */
 int clspmv_set_option(const char* name, long long value);
 int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info);
 int clspmv_part_free(clspmv_part_t p);

@@ -183,3 +183,29 @@ def test_e4m3_decode_every_code(ctx):
     y = _run(ctx, part, coo, np.ones(n, np.float32), np.zeros(n, np.float32))
     assert np.array_equal(y, vals)
     part.free()
+
+
+@pytest.mark.parametrize("codec", CODECS)
+def test_cocktail_value_codec(ctx, codec):
+    # the Cocktail row-run parts with a mixed-precision variant (sliced BSR,
+    # x-tiled) take the codec; the others stay FP32
+    import ctypes as C
+
+    from clspmv_amd import cocktail as CK
+    from tests.test_gpu_cocktail import _blocks_scattered
+
+    coo = _blocks_scattered(41)
+    mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(CK.REFERENCE_TABLES))
+    x, y0 = M.gen_vector(coo.width, 29), M.gen_vector(coo.height, 30)
+    L.check(L.lib().clspmv_set_option(b"cocktail_value_codec", C.c_longlong(codec)), "set_option")
+    try:
+        gpu = CK.GpuCocktail(ctx, mat, x, y0)
+    finally:
+        L.check(L.lib().clspmv_set_option(b"cocktail_value_codec", C.c_longlong(0)), "set_option")
+    names = [p["kernel"] for p in gpu.part_infos()]
+    assert names[0].startswith("bsr44_mixed_") and all("mixed" in n or n.startswith(("xtile_", "csr_sliced"))
+                                                       for n in names), names
+    gpu.do_spmv(1)
+    assert _err(gpu.result_to_cpu(), coo, x, y0) <= BOUND[codec]
+    gpu.free()
+
