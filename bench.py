@@ -261,8 +261,7 @@ def mixed_c3(ctx, args, work, per):
     mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(CK.MI355X_TABLES))
     rec = {}
     ms32 = None
-    for codec, label in ((0, "fp32"), (L.VAL_BF16X2, "bf16x2"), (L.VAL_FP16_ROW, "fp16row"),
-                         (L.VAL_FP8_SLICE, "fp8slice")):
+    for codec, label in ((0, "fp32"),) + _codec_labels(L):
         L.check(L.lib().clspmv_set_option(b"cocktail_value_codec", C.c_longlong(codec)), "set_option")
         try:
             gpu = CK.GpuCocktail(ctx, mat, x, np.zeros(coo.height, np.float32))
@@ -287,15 +286,21 @@ def mixed_c3(ctx, args, work, per):
             log(f"c3 c2 {rec[label]['kernel']}: {ms * 1e3:.1f} us ({ms32 / ms:.2f}x FP32 Cocktail), "
                 f"max rel err {e_max:.2e}, norm rel err {e_norm:.2e}")
         gpu.free()
+    _best_within(rec, 1e-4)
     mat.free()
     coo.free()
     out["c2"] = rec
     return out
 
 
+def _codec_labels(L):
+    return ((L.VAL_BF16X2, "bf16x2"), (L.VAL_FP16_ROW, "fp16row"), (L.VAL_FP8_SLICE, "fp8slice"),
+            (L.VAL_I16_ROW, "i16row"))
+
+
 def _mixed_codecs(ctx, M, L, rec, fmt, mat, coo, w, ms32, y64, srow, time_part, run_once, name=None):
     """Adds one record per value codec to rec (time, bytes, speedup, errors)."""
-    for codec, label in ((L.VAL_BF16X2, "bf16x2"), (L.VAL_FP16_ROW, "fp16row"), (L.VAL_FP8_SLICE, "fp8slice")):
+    for codec, label in _codec_labels(L):
         part = M.MixedPart(ctx, fmt, mat, codec, True)
         ms = time_part(part, w["dx"], w["dy"])
         e_max, e_norm = _errors(run_once(part, w["dx"], coo.height), y64, srow)
@@ -306,6 +311,16 @@ def _mixed_codecs(ctx, M, L, rec, fmt, mat, coo, w, ms32, y64, srow, time_part, 
         log(f"c3 {name or w.get('name')} {part.info['kernel']}: {ms * 1e3:.1f} us ({ms32 / ms:.2f}x FP32), "
             f"{part.info['stored_bytes'] / 1e6:.1f} MB, max rel err {e_max:.2e}, norm rel err {e_norm:.2e}")
         part.free()
+    _best_within(rec, 1e-4)
+
+
+def _best_within(rec, tol):
+    """rec["best_max_err_1e-4"]: the fastest codec whose componentwise error
+    is at most tol, with its speedup over FP32."""
+    ok = [(r["ms"], lab) for lab, r in rec.items() if lab != "fp32" and r["max_rel_err"] <= tol]
+    if ok:
+        ms, lab = min(ok)
+        rec["best_max_err_1e-4"] = {"codec": lab, "speedup_vs_fp32": rec[lab]["speedup_vs_fp32"]}
 
 
 def batched_c4(ctx, args):

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libclspmv.so")
 
 OK = 0
-VAL_BF16X2, VAL_FP16_ROW, VAL_FP8_SLICE = 1, 2, 3
+VAL_BF16X2, VAL_FP16_ROW, VAL_FP8_SLICE, VAL_I16_ROW = 1, 2, 3, 4
 E_INVALID, E_TOO_LARGE, E_PADDING, E_HIP, E_NOMEM, E_IO, E_SORT, E_UNSUPPORTED, E_NODEVICE = (
     -1, -2, -3, -4, -5, -6, -7, -8, -9)
 ERROR_NAMES = {0: "OK", -1: "E_INVALID", -2: "E_TOO_LARGE", -3: "E_PADDING", -4: "E_HIP",

@@ -495,7 +495,7 @@ __attribute__((visibility("hidden"))) int clspmv_cocktail_option(const char* nam
         return CLSPMV_OK;
     }
     if (name && std::strcmp(name, "cocktail_value_codec") == 0) {
-        if (value < 0 || value > CLSPMV_VAL_FP8_SLICE) return CLSPMV_E_INVALID;
+        if (value < 0 || value > CLSPMV_VAL_I16_ROW) return CLSPMV_E_INVALID;
         g_run_codec = (int)value;
         return CLSPMV_OK;
     }

@@ -9,7 +9,11 @@
 //   kCodecFp8Slice  v = s * q, q FP8 E4M3 (OCP: bias 7, 3 mantissa bits,
 //                   largest finite 448), one FP32 scale per slice: 256 rows
 //                   of a diagonal (DIA) or 64 block rows of a block element
-//                   (BSR): 1 byte per value.
+//                   (BSR): 1 byte per value;
+//   kCodecI16Row    v = s_row * q, q a signed 16-bit integer (|q| <= 32767),
+//                   s_row = (largest magnitude of the row) / 32767, applied
+//                   to the row sums: 2 bytes per value, every value within
+//                   s_row / 2 of its FP32 value.
 // DIA: 1, 2 or 4 consecutive rows per lane, so that every lane loads one
 // 32-bit word of values per diagonal.  BSR: the sliced layout of
 // bsr_sliced_kernel with the encoded values and, optionally, block columns
@@ -63,12 +67,12 @@ __device__ __forceinline__ void store_vec(float* p, const float (&v)[R]) {
 // values per diagonal (1 BF16 hi+lo word, 2 FP16 or 4 FP8 values).
 template <int CODEC>
 constexpr int dia_rows_per_lane() {
-    return CODEC == kCodecFp8Slice ? 4 : CODEC == kCodecFp16Row ? 2 : 1;
+    return CODEC == kCodecFp8Slice ? 4 : codec_row_scaled(CODEC) ? 2 : 1;
 }
 
-// The R values of rows row .. row+R-1 in one word, decoded; FP16 values are
-// returned unscaled (the row scale is applied to the row sums), FP8 values
-// times the slice scale s.
+// The R values of rows row .. row+R-1 in one word, decoded; FP16 and int16
+// values are returned unscaled (the row scale is applied to the row sums),
+// FP8 values times the slice scale s.
 template <int CODEC, int R>
 __device__ __forceinline__ void dia_decode(unsigned w, float s, float (&v)[R]) {
     if constexpr (CODEC == kCodecBf16x2) {
@@ -76,6 +80,9 @@ __device__ __forceinline__ void dia_decode(unsigned w, float s, float (&v)[R]) {
     } else if constexpr (CODEC == kCodecFp16Row) {
         v[0] = f16_value((uint16_t)(w & 0xffffu));
         v[1] = f16_value((uint16_t)(w >> 16));
+    } else if constexpr (CODEC == kCodecI16Row) {
+        v[0] = i16_lo(w);
+        v[1] = i16_hi(w);
     } else {
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = s * e4m3_value((w >> (8 * j)) & 0xffu);
@@ -159,9 +166,9 @@ __global__ __launch_bounds__(256) void dia_mixed_kernel(const int* __restrict__ 
         }
     }
     float rs[R];
-    if (CODEC == kCodecFp16Row) load_vec<R>(scale + row, rs);  // the scale array has L >= row + R entries
+    if (codec_row_scaled(CODEC)) load_vec<R>(scale + row, rs);  // the scale array has L >= row + R entries
 #pragma unroll
-    for (int j = 0; j < R; ++j) yv[j] += CODEC == kCodecFp16Row ? a[j] * rs[j] : a[j];
+    for (int j = 0; j < R; ++j) yv[j] += codec_row_scaled(CODEC) ? a[j] * rs[j] : a[j];
     if (full) {
         store_vec<R>(y + row, yv);
     } else {
@@ -172,12 +179,13 @@ __global__ __launch_bounds__(256) void dia_mixed_kernel(const int* __restrict__ 
 }
 
 // Value e of the block in slot idx, decoded (FP8: times the scale of element
-// e in slice s; FP16: the row scale is applied by the caller).
+// e in slice s; FP16, int16: the row scale is applied by the caller).
 template <int CODEC>
 __device__ __forceinline__ float bsr_value(const void* __restrict__ data, const float* __restrict__ scale, size_t i,
                                           int e, int nsl, int s) {
     if (CODEC == kCodecBf16x2) return bf16x2_value(ld_stream(reinterpret_cast<const uint32_t*>(data) + i));
     if (CODEC == kCodecFp16Row) return f16_value(ld_narrow(reinterpret_cast<const uint16_t*>(data) + i));
+    if (CODEC == kCodecI16Row) return (float)ld_narrow(reinterpret_cast<const int16_t*>(data) + i);
     return scale[(size_t)e * nsl + s] * e4m3_value(ld_narrow(reinterpret_cast<const uint8_t*>(data) + i));
 }
 
@@ -248,11 +256,11 @@ __global__ __launch_bounds__(256) void bsr_mixed_kernel(const int* __restrict__ 
                 acc[h] = fmaf(bsr_value<CODEC>(data, scale, (size_t)(h * BW + w) * stride + idx, h * BW + w, nsl, s),
                               xv[w], acc[h]);
     }
-    const float rs = CODEC == kCodecFp16Row ? scale[r] : 1.f;
+    const float rs = codec_row_scaled(CODEC) ? scale[r] : 1.f;
 #pragma unroll
     for (int h = 0; h < BH; ++h) {
         const int row = r * BH + h;
-        if (row < rows) y[row] += CODEC == kCodecFp16Row ? rs * acc[h] : acc[h];
+        if (row < rows) y[row] += codec_row_scaled(CODEC) ? rs * acc[h] : acc[h];
     }
 }
 
@@ -268,6 +276,7 @@ int launch_dia_mixed(const DevDiaMixed& m, const float* x, float* y, hipStream_t
     switch (m.codec) {
         case kCodecBf16x2: CLSPMV_DIA_MIXED(kCodecBf16x2); break;
         case kCodecFp16Row: CLSPMV_DIA_MIXED(kCodecFp16Row); break;
+        case kCodecI16Row: CLSPMV_DIA_MIXED(kCodecI16Row); break;
         default: CLSPMV_DIA_MIXED(kCodecFp8Slice); break;
     }
 #undef CLSPMV_DIA_MIXED
@@ -283,10 +292,12 @@ static void bsr_mixed_launch(const DevBsrMixed& m, const float* x, float* y, hip
     if (m.delta) {
         if (m.codec == kCodecBf16x2) CLSPMV_BSR_MIXED(kCodecBf16x2, true);
         else if (m.codec == kCodecFp16Row) CLSPMV_BSR_MIXED(kCodecFp16Row, true);
+        else if (m.codec == kCodecI16Row) CLSPMV_BSR_MIXED(kCodecI16Row, true);
         else CLSPMV_BSR_MIXED(kCodecFp8Slice, true);
     } else {
         if (m.codec == kCodecBf16x2) CLSPMV_BSR_MIXED(kCodecBf16x2, false);
         else if (m.codec == kCodecFp16Row) CLSPMV_BSR_MIXED(kCodecFp16Row, false);
+        else if (m.codec == kCodecI16Row) CLSPMV_BSR_MIXED(kCodecI16Row, false);
         else CLSPMV_BSR_MIXED(kCodecFp8Slice, false);
     }
 #undef CLSPMV_BSR_MIXED

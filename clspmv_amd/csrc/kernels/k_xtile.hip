@@ -165,6 +165,7 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
 template <int CODEC> struct XtVal { typedef f32x2 T; };
 template <> struct XtVal<kCodecBf16x2> { typedef u32x2 T; };
 template <> struct XtVal<kCodecFp16Row> { typedef unsigned T; };        // entry 0 in the low half
+template <> struct XtVal<kCodecI16Row> { typedef unsigned T; };         // entry 0 in the low half
 template <> struct XtVal<kCodecFp8Slice> { typedef unsigned short T; };  // entry 0 in the low byte
 
 template <int CODEC>
@@ -173,6 +174,8 @@ __device__ __forceinline__ f32x2 xt_decode(typename XtVal<CODEC>::T v, float s) 
         return f32x2{bf16x2_value(v.x), bf16x2_value(v.y)};
     } else if constexpr (CODEC == kCodecFp16Row) {
         return f32x2{f16_value((uint16_t)(v & 0xffffu)), f16_value((uint16_t)(v >> 16))};
+    } else if constexpr (CODEC == kCodecI16Row) {
+        return f32x2{i16_lo(v), i16_hi(v)};
     } else if constexpr (CODEC == kCodecFp8Slice) {
         return f32x2{s * e4m3_value(v & 0xffu), s * e4m3_value((unsigned)v >> 8)};
     } else {
@@ -335,7 +338,7 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     __syncthreads();
     if (st.pending && tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
     __syncthreads();
-    if constexpr (CODEC == kCodecFp16Row)
+    if constexpr (codec_row_scaled(CODEC))
         for (int r = tid; r < nr; r += kXtThreads) y_l[r] *= a.vscale[r0 + r];
     if (out) {
         for (int r = tid; r < nr; r += kXtThreads) out[r] = y_l[r];
@@ -366,6 +369,7 @@ int launch_xtile(const DevXtile& m, const float* x, float* y, hipStream_t st) {
     switch (m.codec) {
         case kCodecBf16x2: CLSPMV_XTILE(kCodecBf16x2); break;
         case kCodecFp16Row: CLSPMV_XTILE(kCodecFp16Row); break;
+        case kCodecI16Row: CLSPMV_XTILE(kCodecI16Row); break;
         case kCodecFp8Slice: CLSPMV_XTILE(kCodecFp8Slice); break;
         default: CLSPMV_XTILE(0); break;
     }

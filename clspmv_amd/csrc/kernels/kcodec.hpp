@@ -12,6 +12,10 @@ __device__ __forceinline__ float bf16x2_value(uint32_t u) {
     return __uint_as_float(u & 0xffff0000u) + __uint_as_float(u << 16);
 }
 __device__ __forceinline__ float f16_value(uint16_t h) { return __half2float(__ushort_as_half(h)); }
+// Signed 16-bit fixed point: the low (lo) or high (hi) half of a word,
+// sign-extended and converted (exact: |q| <= 32767).
+__device__ __forceinline__ float i16_lo(uint32_t w) { return (float)((int)(w << 16) >> 16); }
+__device__ __forceinline__ float i16_hi(uint32_t w) { return (float)((int)w >> 16); }
 // E4M3: normal (e >= 1) 2^(e-7) * (1 + m/8), subnormal (e = 0) m * 2^-9;
 // the caller's scale absorbs the magnitude.  Branch-free: the 7 magnitude
 // bits shifted into place form the FP32 2^(e-127) * (1 + m/8) for e >= 1 and

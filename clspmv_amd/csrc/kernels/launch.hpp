@@ -102,6 +102,10 @@ constexpr int kBsrSlice = 64;  // block rows per slice of the sliced BSR layout
 constexpr int kCodecBf16x2 = 1;
 constexpr int kCodecFp16Row = 2;
 constexpr int kCodecFp8Slice = 3;
+constexpr int kCodecI16Row = 4;
+// Codecs whose values are divided by one scale per row (per block row for
+// BSR); the kernels multiply the row sums by it.
+constexpr bool codec_row_scaled(int c) { return c == kCodecFp16Row || c == kCodecI16Row; }
 
 // LDS x-tiled layout (k_xtile.hip): row blocks of <= kXtRows rows, column
 // slices of kXtCols columns, entries packed (row_in_block << 16 |

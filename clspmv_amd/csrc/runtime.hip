@@ -149,9 +149,14 @@ int make_part(clspmv_ctx_t ctx, clspmv_part_t* out, F build) {
     return CLSPMV_OK;
 }
 
-constexpr int kMethodMixed = 16;  // part method = 16 + codec (+ 4 with int16 delta columns)
+constexpr int kMethodMixed = 16;  // part method = 16 + codec (+ 8 with int16 delta columns)
 
-bool codec_ok(int codec) { return codec == kCodecBf16x2 || codec == kCodecFp16Row || codec == kCodecFp8Slice; }
+static_assert(kCodecBf16x2 == CLSPMV_VAL_BF16X2 && kCodecFp16Row == CLSPMV_VAL_FP16_ROW &&
+                  kCodecFp8Slice == CLSPMV_VAL_FP8_SLICE && kCodecI16Row == CLSPMV_VAL_I16_ROW,
+              "codec ids are the ABI's CLSPMV_VAL_* values");
+bool codec_ok(int codec) {
+    return codec == kCodecBf16x2 || codec == kCodecFp16Row || codec == kCodecFp8Slice || codec == kCodecI16Row;
+}
 
 // BF16 hi + lo split of v: (bf16(v) << 16) | bf16(v - bf16(v)).
 uint32_t bf16x2_word(float v) {
@@ -165,9 +170,31 @@ float fp8_scale(float amax) { return amax > 0.f ? amax / 448.0f : 1.0f; }
 // FP16 row scale: a power of two (so v / s is exact) that puts the row's
 // largest magnitude in [2^14, 2^15), well inside FP16's normal range.
 float fp16_scale(float amax) { return amax > 0.f ? std::ldexp(1.0f, std::ilogb(amax) - 14) : 1.0f; }
+// Row scale of the int16 codec: the row's largest magnitude maps to 32767.
+float i16_scale(float amax) { return amax > 0.f ? amax / 32767.0f : 1.0f; }
+float row_scale(int codec, float amax) { return codec == kCodecI16Row ? i16_scale(amax) : fp16_scale(amax); }
+// The int16 codec holds finite values only: a row with an infinite or NaN
+// value is refused rather than saturated.
+int check_row_scales(int codec, const std::vector<float>& s) {
+    if (codec != kCodecI16Row) return CLSPMV_OK;
+    for (float v : s)
+        if (!std::isfinite(v)) {
+            set_error("the int16 value codec needs finite matrix values");
+            return CLSPMV_E_INVALID;
+        }
+    return CLSPMV_OK;
+}
+// v / s for a row-scaled codec, as the stored 16 bits.
+uint16_t row_encode(int codec, float v_over_s) {
+    if (codec == kCodecFp16Row) return f32_to_f16(v_over_s);
+    return (uint16_t)(int16_t)std::max(-32767.0f, std::min(32767.0f, std::nearbyint(v_over_s)));
+}
 
 const char* codec_name(int codec) {
-    return codec == kCodecBf16x2 ? "bf16x2" : codec == kCodecFp16Row ? "fp16row" : "fp8slice";
+    return codec == kCodecBf16x2  ? "bf16x2"
+           : codec == kCodecFp16Row ? "fp16row"
+           : codec == kCodecI16Row  ? "i16row"
+                                    : "fp8slice";
 }
 
 int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
@@ -591,7 +618,8 @@ int g_xt_max_groups = kXtMaxGroups;
 // Within a block the entries are bucketed by column slice in row order, so
 // each tile is sorted by (row, col) when the rows are.
 // `codec` (0 or a kCodec* value) encodes the values: FP16 divided by a
-// power-of-two scale per row, FP8 with one scale per 128-entry chunk.
+// power-of-two scale per row, int16 fixed point with one scale per row, FP8
+// with one scale per 128-entry chunk.
 int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
                 int cols, int codec) {
     const long long nnz = rows ? rp[rows] : 0;
@@ -663,14 +691,15 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
     std::vector<unsigned> idx(std::max<size_t>(total, 1), kXtPadRow << 16);
     std::vector<float> val(std::max<size_t>(total, 1), 0.f);
     std::vector<float> rscale;
-    if (codec == kCodecFp16Row) {
+    if (codec_row_scaled(codec)) {
         rscale.assign((size_t)std::max(rows, 1), 1.0f);
 #pragma omp parallel for schedule(static) num_threads(omp_threads())
         for (int r = 0; r < rows; ++r) {
             float a = 0.f;
             for (long long k = rp[r]; k < rp[r + 1]; ++k) a = std::max(a, std::fabs(cv[k]));
-            rscale[(size_t)r] = fp16_scale(a);
+            rscale[(size_t)r] = row_scale(codec, a);
         }
+        if (int e = check_row_scales(codec, rscale)) return e;
     }
 #pragma omp parallel num_threads(omp_threads())
     {
@@ -708,10 +737,10 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
 #pragma omp parallel for schedule(static) num_threads(omp_threads())
         for (long long i = 0; i < (long long)val.size(); ++i) w[(size_t)i] = bf16x2_word(val[(size_t)i]);
         d.val = p->add(ctx, w.data(), w.size() * 4, &rc);
-    } else if (codec == kCodecFp16Row) {
+    } else if (codec_row_scaled(codec)) {
         std::vector<uint16_t> h(val.size());
 #pragma omp parallel for schedule(static) num_threads(omp_threads())
-        for (long long i = 0; i < (long long)val.size(); ++i) h[(size_t)i] = f32_to_f16(val[(size_t)i]);
+        for (long long i = 0; i < (long long)val.size(); ++i) h[(size_t)i] = row_encode(codec, val[(size_t)i]);
         d.val = p->add(ctx, h.data(), std::max<size_t>(h.size(), 2) * 2, &rc);
         vscale.swap(rscale);
     } else if (codec == kCodecFp8Slice) {
@@ -1177,17 +1206,18 @@ int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int code
 #pragma omp parallel for schedule(static) num_threads(omp_threads())
             for (long long i = 0; i < (long long)n; ++i) w[(size_t)i] = bf16x2_word(m->dia_data[i]);
             enc.assign(reinterpret_cast<unsigned char*>(w.data()), reinterpret_cast<unsigned char*>(w.data()) + n * 4);
-        } else if (codec == kCodecFp16Row) {
+        } else if (codec_row_scaled(codec)) {
             scale.assign((size_t)L, 1.0f);
             for (int r = 0; r < L; ++r) {
                 float a = 0.f;
                 for (int d = 0; d < nd; ++d) a = std::max(a, std::fabs(m->dia_data[(size_t)d * L + r]));
-                scale[(size_t)r] = fp16_scale(a);
+                scale[(size_t)r] = row_scale(codec, a);
             }
+            if (int e = check_row_scales(codec, scale)) return e;
             std::vector<uint16_t> h(n);
 #pragma omp parallel for schedule(static) num_threads(omp_threads())
             for (long long i = 0; i < (long long)n; ++i)
-                h[(size_t)i] = f32_to_f16(m->dia_data[i] / scale[(size_t)(i % L)]);
+                h[(size_t)i] = row_encode(codec, m->dia_data[i] / scale[(size_t)(i % L)]);
             enc.assign(reinterpret_cast<unsigned char*>(h.data()), reinterpret_cast<unsigned char*>(h.data()) + n * 2);
         } else {
             scale.assign((size_t)nd * nsl, 1.0f);
@@ -1306,7 +1336,7 @@ int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int code
             std::vector<uint32_t> w(nv);
             for (size_t i = 0; i < nv; ++i) w[i] = bf16x2_word(sdat[i]);
             enc.assign(reinterpret_cast<unsigned char*>(w.data()), reinterpret_cast<unsigned char*>(w.data()) + nv * 4);
-        } else if (codec == kCodecFp16Row) {
+        } else if (codec_row_scaled(codec)) {
             scale.assign((size_t)std::max(1, brows), 1.0f);
             for (int r = 0; r < brows; ++r) {
                 const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
@@ -1314,15 +1344,16 @@ int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int code
                 for (int k = 0; k < width; ++k)
                     for (unsigned q = 0; q < bs; ++q)
                         amax = std::max(amax, std::fabs(sdat[(size_t)q * sstride + (size_t)sp[s] + (size_t)k * S + t]));
-                scale[(size_t)r] = fp16_scale(amax);
+                scale[(size_t)r] = row_scale(codec, amax);
             }
+            if (int e = check_row_scales(codec, scale)) return e;
             std::vector<uint16_t> h(nv, 0);
             for (int r = 0; r < brows; ++r) {
                 const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
                 for (int k = 0; k < width; ++k)
                     for (unsigned q = 0; q < bs; ++q) {
                         const size_t i = (size_t)q * sstride + (size_t)sp[s] + (size_t)k * S + t;
-                        h[i] = f32_to_f16(sdat[i] / scale[(size_t)r]);
+                        h[i] = row_encode(codec, sdat[i] / scale[(size_t)r]);
                     }
             }
             enc.assign(reinterpret_cast<unsigned char*>(h.data()), reinterpret_cast<unsigned char*>(h.data()) + nv * 2);
@@ -1370,7 +1401,7 @@ int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int code
         p->mixed = true;
         char name[48];
         std::snprintf(name, sizeof(name), "bsr%u%u_mixed_%s%s", bh, bw, codec_name(codec), delta ? "_d16" : "");
-        fill_info(p, CLSPMV_FMT_BSR, kMethodMixed + codec + (delta ? 4 : 0), rows, cols, m->matinfo.nnz, name, 1);
+        fill_info(p, CLSPMV_FMT_BSR, kMethodMixed + codec + (delta ? 8 : 0), rows, cols, m->matinfo.nnz, name, 1);
         return CLSPMV_OK;
     });
 }

@@ -336,15 +336,21 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
  *                        rows of a diagonal, 64 block rows of a block
  *                        element, or one 128-entry chunk of the x-tiled
  *                        layout (1 B)
+ *   CLSPMV_VAL_I16_ROW   signed 16-bit fixed point times one FP32 scale per
+ *                        row (block row), the row's largest magnitude over
+ *                        32767: every value within half a step of its FP32
+ *                        value (2 B; finite values only, else
+ *                        CLSPMV_E_INVALID)
  * accumulated in FP32.  DIA: dia_length_aligned must be a multiple of 256
  * (clspmv_coo2dia with alignment 256).  BSR: width >= block width.  BSR r x r (2 <= r <= 4), sliced
  * layout; delta_cols != 0 stores block columns as int16 steps within a block
- * row when every step fits (else int32; info.method tells: 16 + codec, + 4
+ * row when every step fits (else int32; info.method tells: 16 + codec, + 8
  * with int16 columns).  CSR: the LDS x-tiled layout (whose entries pack row
  * and column in 4 bytes) with encoded values; info.method = 16 + codec. */
 #define CLSPMV_VAL_BF16X2     1
 #define CLSPMV_VAL_FP16_ROW   2
 #define CLSPMV_VAL_FP8_SLICE  3
+#define CLSPMV_VAL_I16_ROW    4
 int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int codec, clspmv_part_t* p);
 int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int codec, int delta_cols, clspmv_part_t* p);
 int clspmv_part_csr_mixed(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int codec, clspmv_part_t* p);

@@ -1,8 +1,10 @@
 """Mixed-precision DIA and BSR parts against the FP64 oracle.  Error metric
 per element: |y - y64| / max(|y64|, S) with S = sum_j |a_ij x_j| (plus |y0|):
 bounded by 2e-5 for BF16 hi+lo (about 16 significant bits), 5e-4 for FP16
-with a per-row scale (11 bits) and 0.08 for FP8 E4M3 with a per-slice scale
-(4 bits: half an ulp is 1/16).  int16 delta columns give results bitwise
+with a per-row scale (11 bits), 0.08 for FP8 E4M3 with a per-slice scale
+(4 bits: half an ulp is 1/16) and, for int16 fixed point with a per-row
+scale s = amax / 32767, by the codec's own bound sum_j (s/2) |x_j| / S plus
+FP32 rounding.  int16 delta columns give results bitwise
 equal to int32 columns, and fall back to int32 when a step does not fit."""
 import numpy as np
 import pytest
@@ -13,8 +15,8 @@ from oracle import oracle as O
 from tests.helpers import rand_coo
 
 pytestmark = pytest.mark.gpu
-BOUND = {L.VAL_BF16X2: 2e-5, L.VAL_FP16_ROW: 5e-4, L.VAL_FP8_SLICE: 0.08}
-CODECS = [L.VAL_BF16X2, L.VAL_FP16_ROW, L.VAL_FP8_SLICE]
+BOUND = {L.VAL_BF16X2: 2e-5, L.VAL_FP16_ROW: 5e-4, L.VAL_FP8_SLICE: 0.08, L.VAL_I16_ROW: 2e-5}
+CODECS = [L.VAL_BF16X2, L.VAL_FP16_ROW, L.VAL_FP8_SLICE, L.VAL_I16_ROW]
 
 
 def _run(ctx, part, coo, x, y0):
@@ -31,6 +33,36 @@ def _err(y, coo, x, y0):
     y64 = O.spmv_f64(coo.row_id, coo.col_id, coo.data, x, coo.height) + y0.astype(np.float64)
     S = O.abs_row_sum(coo.row_id, coo.col_id, coo.data, x, coo.height) + np.abs(y0)
     return float(np.max(np.abs(y - y64) / np.maximum(np.maximum(np.abs(y64), S), 1e-30)))
+
+
+def _row_amax(coo, group=1, window=0):
+    """Largest |a_ij| of each row's scale group: the row (group 1), the
+    aligned block row of `group` rows, or any row within `window` rows."""
+    a = np.zeros(coo.height, np.float64)
+    np.maximum.at(a, coo.row_id, np.abs(coo.data).astype(np.float64))
+    if group > 1:
+        n = (coo.height + group - 1) // group
+        a = np.repeat(np.pad(a, (0, n * group - coo.height)).reshape(n, group).max(1), group)[:coo.height]
+    for k in range(1, window + 1):
+        a = np.maximum(a, np.maximum(np.pad(a[k:], (0, k)), np.pad(a[:-k], (k, 0))))
+    return a
+
+
+def _check(codec, y, coo, x, y0, group=1, window=0):
+    """The codec's bound.  int16 rows: |y - y64| <= (s/2) sum_j |x_j| plus
+    1e-5 (S + |y0|) for the FP32 arithmetic, s = amax / 32767 of the row's
+    scale group; the others: the componentwise bound BOUND[codec]."""
+    e = _err(y, coo, x, y0)
+    if codec != L.VAL_I16_ROW:
+        assert e <= BOUND[codec], (codec, e)
+        return e
+    y64 = O.spmv_f64(coo.row_id, coo.col_id, coo.data, x, coo.height) + y0.astype(np.float64)
+    S = O.abs_row_sum(coo.row_id, coo.col_id, coo.data, x, coo.height) + np.abs(y0)
+    xs = np.bincount(coo.row_id, np.abs(x[coo.col_id]).astype(np.float64), minlength=coo.height)
+    bound = _row_amax(coo, group, window) / 32767 / 2 * xs + 1e-5 * S
+    d = np.abs(y - y64)
+    assert np.all(d <= bound), (int(np.argmax(d - bound)), float(np.max(d - bound)))
+    return e
 
 
 def _rect_band(h, w, offsets, seed):
@@ -55,8 +87,7 @@ def test_dia_mixed(ctx, codec, name, make):
     x, y0 = M.gen_vector(coo.width, 1), M.gen_vector(coo.height, 2)
     part = M.MixedPart(ctx, L.FMT_DIA, M.coo2dia(coo, 256), codec)
     assert part.info["method"] == 16 + codec
-    e = _err(_run(ctx, part, coo, x, y0), coo, x, y0)
-    assert e <= BOUND[codec], (name, codec, e)
+    _check(codec, _run(ctx, part, coo, x, y0), coo, x, y0)
     part.free()
 
 
@@ -73,12 +104,11 @@ def test_bsr_mixed(ctx, codec, b, name, make):
     bsr = M.coo2bsr(coo, b, b)
     p16 = M.MixedPart(ctx, L.FMT_BSR, bsr, codec, True)
     p32 = M.MixedPart(ctx, L.FMT_BSR, bsr, codec, False)
-    assert p16.info["method"] == 16 + codec + 4 and p32.info["method"] == 16 + codec
+    assert p16.info["method"] == 16 + codec + 8 and p32.info["method"] == 16 + codec
     ya = _run(ctx, p16, coo, x, y0)
     yb = _run(ctx, p32, coo, x, y0)
     assert np.array_equal(ya.view(np.uint32), yb.view(np.uint32))
-    e = _err(ya, coo, x, y0)
-    assert e <= BOUND[codec], (name, b, codec, e)
+    _check(codec, ya, coo, x, y0, group=b)
     p16.free()
     p32.free()
 
@@ -123,8 +153,7 @@ def test_csr_mixed_xtile(ctx, codec, name, make):
     x, y0 = M.gen_vector(coo.width, 6), M.gen_vector(coo.height, 7)
     part = M.MixedPart(ctx, L.FMT_CSR_VECTOR, M.coo2csr(coo), codec)
     assert part.info["method"] == 16 + codec and part.info["kernel"].startswith("xtile_")
-    e = _err(_run(ctx, part, coo, x, y0), coo, x, y0)
-    assert e <= BOUND[codec], (name, codec, e)
+    _check(codec, _run(ctx, part, coo, x, y0), coo, x, y0)
     part.free()
 
 
@@ -145,9 +174,43 @@ def test_csr_mixed_exact_values_match_fp32(ctx, codec, name, make):
     plain.free()
 
 
+@pytest.mark.parametrize("name,make", CSR_CASES, ids=[c[0] for c in CSR_CASES])
+def test_csr_i16_exact_values_match_fp32(ctx, name, make):
+    # Integer values in [-32767, 32767] with +-32767 in every nonempty row:
+    # the row scale is exactly 1, the stored integers are the values, so y
+    # must be bitwise that of the FP32 x-tiled part (pins the sign extension
+    # of both 16-bit halves).
+    coo = make()
+    rng = np.random.default_rng(21)
+    v = rng.integers(-32767, 32768, coo.nnz).astype(np.float32)
+    first = np.flatnonzero(np.r_[True, coo.row_id[1:] != coo.row_id[:-1]]) if coo.nnz else np.zeros(0, np.int64)
+    v[first] = np.where(rng.random(first.size) < 0.5, 32767.0, -32767.0)
+    coo = M.CooMatrix.from_arrays(coo.row_id.copy(), coo.col_id.copy(), v, coo.width, coo.height)
+    csr = M.coo2csr(coo)
+    x, y0 = M.gen_vector(coo.width, 10), M.gen_vector(coo.height, 11)
+    mixed = M.MixedPart(ctx, L.FMT_CSR_VECTOR, csr, L.VAL_I16_ROW)
+    plain = M.Part(ctx, L.FMT_CSR_VECTOR, csr, 4)
+    ya, yb = _run(ctx, mixed, coo, x, y0), _run(ctx, plain, coo, x, y0)
+    assert np.array_equal(ya.view(np.uint32), yb.view(np.uint32)), name
+    mixed.free()
+    plain.free()
+
+
+@pytest.mark.parametrize("fmt", ["dia", "bsr", "csr"])
+def test_i16_refuses_non_finite(ctx, fmt):
+    coo = M.gen_laplacian2d(32, 32)
+    v = coo.data.copy()
+    v[100] = np.inf
+    coo = M.CooMatrix.from_arrays(coo.row_id.copy(), coo.col_id.copy(), v, coo.width, coo.height)
+    mat = {"dia": lambda: M.coo2dia(coo, 256), "bsr": lambda: M.coo2bsr(coo, 2, 2), "csr": lambda: M.coo2csr(coo)}[fmt]()
+    f = {"dia": L.FMT_DIA, "bsr": L.FMT_BSR, "csr": L.FMT_CSR_VECTOR}[fmt]
+    with pytest.raises(RuntimeError, match="finite"):
+        M.MixedPart(ctx, f, mat, L.VAL_I16_ROW)
+
+
 def test_csr_mixed_stored_bytes(ctx):
-    # 8 bytes per x-tiled entry in FP32 and BF16 hi+lo, 6 with FP16 (+ row
-    # scales), 5 with FP8 (+ one scale per 128-entry chunk)
+    # 8 bytes per x-tiled entry in FP32 and BF16 hi+lo, 6 with FP16 or int16
+    # (+ row scales), 5 with FP8 (+ one scale per 128-entry chunk)
     coo = M.gen_uniform_rows(4096, 32, 6)  # 128 chunks of 128 entries, no padding
     csr = M.coo2csr(coo)
     plain = M.Part(ctx, L.FMT_CSR_VECTOR, csr, 4)
@@ -155,6 +218,7 @@ def test_csr_mixed_stored_bytes(ctx):
     base = plain.info["stored_bytes"] - 8 * coo.nnz
     assert parts[L.VAL_BF16X2].info["stored_bytes"] == plain.info["stored_bytes"]
     assert parts[L.VAL_FP16_ROW].info["stored_bytes"] == base + 6 * coo.nnz + 4 * coo.height
+    assert parts[L.VAL_I16_ROW].info["stored_bytes"] == base + 6 * coo.nnz + 4 * coo.height
     assert parts[L.VAL_FP8_SLICE].info["stored_bytes"] == base + 5 * coo.nnz + 4 * (coo.nnz // 128)
     for p in list(parts.values()) + [plain]:
         p.free()
@@ -206,6 +270,6 @@ def test_cocktail_value_codec(ctx, codec):
     assert names[0].startswith("bsr44_mixed_") and all("mixed" in n or n.startswith(("xtile_", "csr_sliced"))
                                                        for n in names), names
     gpu.do_spmv(1)
-    assert _err(gpu.result_to_cpu(), coo, x, y0) <= BOUND[codec]
+    _check(codec, gpu.result_to_cpu(), coo, x, y0, window=3)
     gpu.free()
 
