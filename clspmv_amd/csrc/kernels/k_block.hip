@@ -173,13 +173,34 @@ __global__ __launch_bounds__(256) void bsr_kernel(const int* __restrict__ rp, co
 // Blocks go G = bsr_group(BH * BW) at a time: their column loads, then all
 // x and value loads of the G blocks, are issued back to back (x through
 // load_xblock: no load sits behind a branch).  Requires cols >= BW.
-template <int BH, int BW>
-__global__ __launch_bounds__(256) void bsr_sliced_kernel(const int* __restrict__ sp, const int* __restrict__ col,
-                                                         const float* __restrict__ data, int stride, int brows,
-                                                         const float* __restrict__ x, float* __restrict__ y,
-                                                         int rows, int cols) {
+// XW: kBsrWinRows-thread workgroups first stage their x window win[] into
+// LDS (the columns of a wide band: one coalesced pass instead of a gather
+// of one cache line per lane and block); each lane's sums are the same
+// operations in the same order either way.
+template <int BH, int BW, bool XW>
+__global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_sliced_kernel(
+    const int* __restrict__ sp, const int* __restrict__ col, const float* __restrict__ data, int stride, int brows,
+    const int2* __restrict__ win, const float* __restrict__ x, float* __restrict__ y, int rows, int cols) {
     constexpr int G = bsr_group(BH * BW);
-    const int r = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    constexpr int T = XW ? kBsrWinRows : 256;
+    const int wg = xcd_tile(blockIdx.x, gridDim.x);
+    const int r = wg * T + threadIdx.x;
+    __shared__ __align__(16) float xs[XW ? kBsrWinFloats : 4];
+    int w0 = 0;
+    if constexpr (XW) {
+        const int2 w = win[wg];
+        w0 = w.x;
+        stage_xwindow<T>(x, w.x, w.y, cols, xs);
+        __syncthreads();
+    }
+    auto xblock = [&](int c, float(&xv)[BW]) {
+        if constexpr (XW) {
+#pragma unroll
+            for (int j = 0; j < BW; ++j) xv[j] = xs[c - w0 + j];
+        } else {
+            load_xblock<BW>(x, c, cols, xv);
+        }
+    };
     if (r >= brows) return;
     const int s = r >> 6, t = r & 63;
     const int b = sp[s];
@@ -194,7 +215,7 @@ __global__ __launch_bounds__(256) void bsr_sliced_kernel(const int* __restrict__
         for (int u = 0; u < G; ++u) c[u] = ld_stream(col + b + ((k + u) << 6) + t);
         float xv[G][BW], v[G][BH * BW];
 #pragma unroll
-        for (int u = 0; u < G; ++u) load_xblock<BW>(x, c[u], cols, xv[u]);
+        for (int u = 0; u < G; ++u) xblock(c[u], xv[u]);
 #pragma unroll
         for (int u = 0; u < G; ++u)
 #pragma unroll
@@ -209,7 +230,7 @@ __global__ __launch_bounds__(256) void bsr_sliced_kernel(const int* __restrict__
     for (; k < width; ++k) {
         const int idx = b + (k << 6) + t;
         float xv[BW];
-        load_xblock<BW>(x, ld_stream(col + idx), cols, xv);
+        xblock(ld_stream(col + idx), xv);
 #pragma unroll
         for (int h = 0; h < BH; ++h)
 #pragma unroll
@@ -315,8 +336,13 @@ static void launch_bsr_l(const DevBsr& m, const float* x, float* y, hipStream_t 
 
 template <int BH, int BW>
 static void launch_bsr_sliced(const DevBsr& m, const float* x, float* y, hipStream_t st) {
-    hipLaunchKernelGGL((dev::bsr_sliced_kernel<BH, BW>), dim3((m.brows + 255) / 256), dim3(256), 0, st, m.row_ptr,
-                       m.col, m.data, m.stride, m.brows, x, y, m.rows, m.cols);
+    if (m.win)
+        hipLaunchKernelGGL((dev::bsr_sliced_kernel<BH, BW, true>), dim3((m.brows + kBsrWinRows - 1) / kBsrWinRows),
+                           dim3(kBsrWinRows), 0, st, m.row_ptr, m.col, m.data, m.stride, m.brows, m.win, x, y, m.rows,
+                           m.cols);
+    else
+        hipLaunchKernelGGL((dev::bsr_sliced_kernel<BH, BW, false>), dim3((m.brows + 255) / 256), dim3(256), 0, st,
+                           m.row_ptr, m.col, m.data, m.stride, m.brows, m.win, x, y, m.rows, m.cols);
 }
 
 int launch_bsr(const DevBsr& m, const float* x, float* y, hipStream_t st) {

@@ -190,16 +190,33 @@ __device__ __forceinline__ float bsr_value(const void* __restrict__ data, const 
 }
 
 // As bsr_sliced_kernel (blocks G at a time, loads issued back to back, x
-// through load_xblock; cols >= BW), with encoded values and optionally
-// int16 column steps: the G steps are loaded together and summed in order.
-template <int BH, int BW, int CODEC, bool DELTA>
-__global__ __launch_bounds__(256) void bsr_mixed_kernel(const int* __restrict__ sp, const void* __restrict__ colv,
-                                                        const int* __restrict__ base, const void* __restrict__ data,
-                                                        const float* __restrict__ scale, int stride, int brows,
-                                                        int nsl, const float* __restrict__ x, float* __restrict__ y,
-                                                        int rows, int cols) {
+// through load_xblock or, XW, from the workgroup's x window staged in LDS;
+// cols >= BW), with encoded values and optionally int16 column steps: the
+// G steps are loaded together and summed in order.
+template <int BH, int BW, int CODEC, bool DELTA, bool XW>
+__global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_mixed_kernel(
+    const int* __restrict__ sp, const void* __restrict__ colv, const int* __restrict__ base,
+    const void* __restrict__ data, const float* __restrict__ scale, int stride, int brows, int nsl,
+    const int2* __restrict__ win, const float* __restrict__ x, float* __restrict__ y, int rows, int cols) {
     constexpr int G = bsr_group(BH * BW);
-    const int r = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    constexpr int T = XW ? kBsrWinRows : 256;
+    const int r = xcd_tile(blockIdx.x, gridDim.x) * T + threadIdx.x;
+    __shared__ __align__(16) float xs[XW ? kBsrWinFloats : 4];
+    int w0 = 0;
+    if constexpr (XW) {
+        const int2 w = win[xcd_tile(blockIdx.x, gridDim.x)];
+        w0 = w.x;
+        stage_xwindow<T>(x, w.x, w.y, cols, xs);
+        __syncthreads();
+    }
+    auto xblock = [&](int c, float(&xv)[BW]) {
+        if constexpr (XW) {
+#pragma unroll
+            for (int j = 0; j < BW; ++j) xv[j] = xs[c - w0 + j];
+        } else {
+            load_xblock<BW>(x, c, cols, xv);
+        }
+    };
     if (r >= brows) return;
     // a wave's 64 block rows are one slice: wave-uniform slice quantities
     const int s = __builtin_amdgcn_readfirstlane(r >> 6), t = r & 63;
@@ -227,7 +244,7 @@ __global__ __launch_bounds__(256) void bsr_mixed_kernel(const int* __restrict__ 
         }
         float xv[G][BW], v[G][BH * BW];
 #pragma unroll
-        for (int u = 0; u < G; ++u) load_xblock<BW>(x, c[u], cols, xv[u]);
+        for (int u = 0; u < G; ++u) xblock(c[u], xv[u]);
 #pragma unroll
         for (int u = 0; u < G; ++u)
 #pragma unroll
@@ -248,7 +265,7 @@ __global__ __launch_bounds__(256) void bsr_mixed_kernel(const int* __restrict__ 
             c = cprev;
         }
         float xv[BW];
-        load_xblock<BW>(x, c, cols, xv);
+        xblock(c, xv);
 #pragma unroll
         for (int h = 0; h < BH; ++h)
 #pragma unroll
@@ -283,12 +300,13 @@ int launch_dia_mixed(const DevDiaMixed& m, const float* x, float* y, hipStream_t
     return (int)hipGetLastError();
 }
 
-template <int BH, int BW>
+template <int BH, int BW, bool XW>
 static void bsr_mixed_launch(const DevBsrMixed& m, const float* x, float* y, hipStream_t st) {
-    const dim3 g((m.brows + 255) / 256), b(256);
-#define CLSPMV_BSR_MIXED(C, D)                                                                                   \
-    hipLaunchKernelGGL((dev::bsr_mixed_kernel<BH, BW, C, D>), g, b, 0, st, m.sp, m.col, m.base, m.data, m.scale, \
-                       m.stride, m.brows, m.nsl, x, y, m.rows, m.cols)
+    constexpr int T = XW ? kBsrWinRows : 256;
+    const dim3 g((m.brows + T - 1) / T), b(T);
+#define CLSPMV_BSR_MIXED(C, D)                                                                                     \
+    hipLaunchKernelGGL((dev::bsr_mixed_kernel<BH, BW, C, D, XW>), g, b, 0, st, m.sp, m.col, m.base, m.data,       \
+                       m.scale, m.stride, m.brows, m.nsl, m.win, x, y, m.rows, m.cols)
     if (m.delta) {
         if (m.codec == kCodecBf16x2) CLSPMV_BSR_MIXED(kCodecBf16x2, true);
         else if (m.codec == kCodecFp16Row) CLSPMV_BSR_MIXED(kCodecFp16Row, true);
@@ -305,10 +323,13 @@ static void bsr_mixed_launch(const DevBsrMixed& m, const float* x, float* y, hip
 
 int launch_bsr_mixed(const DevBsrMixed& m, const float* x, float* y, hipStream_t st) {
     if (m.brows == 0) return 0;
-    switch (m.bh * 10 + m.bw) {
-        case 22: bsr_mixed_launch<2, 2>(m, x, y, st); break;
-        case 33: bsr_mixed_launch<3, 3>(m, x, y, st); break;
-        case 44: bsr_mixed_launch<4, 4>(m, x, y, st); break;
+    switch (m.bh * 10 + m.bw + (m.win ? 100 : 0)) {
+        case 22: bsr_mixed_launch<2, 2, false>(m, x, y, st); break;
+        case 33: bsr_mixed_launch<3, 3, false>(m, x, y, st); break;
+        case 44: bsr_mixed_launch<4, 4, false>(m, x, y, st); break;
+        case 122: bsr_mixed_launch<2, 2, true>(m, x, y, st); break;
+        case 133: bsr_mixed_launch<3, 3, true>(m, x, y, st); break;
+        case 144: bsr_mixed_launch<4, 4, true>(m, x, y, st); break;
         default: return (int)hipErrorInvalidValue;
     }
     return (int)hipGetLastError();

@@ -53,6 +53,45 @@ __device__ __forceinline__ void load_xblock(const float* __restrict__ x, int c, 
     }
 }
 
+// Stages x[w0 .. w0 + n) (zero at and past `cols`) into xs[0 .. n) (xs
+// 16-byte aligned) with the whole workgroup (T threads): 16-byte loads of x
+// at 4-byte alignment, up to 10 per thread in flight (a 40960-float window
+// in one round); requires cols >= 4.  The caller adds the barrier.
+typedef float f32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+template <int T>
+__device__ __forceinline__ void stage_xwindow(const float* __restrict__ x, int w0, int n, int cols, float* xs) {
+    constexpr int U = 10;
+    // loads start at most at lim - 4: past the window (or x) they re-read
+    // its last 16 bytes, a cache hit, rather than x beyond it
+    const int lim = min(w0 + max(n, 4), cols);
+    for (int i0 = 0; i0 < n; i0 += 4 * U * T) {
+        f32x4a4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = w0 + i0 + 4 * (u * T + (int)threadIdx.x);
+            if (i0 + 4 * u * T < n) v[u] = *reinterpret_cast<const f32x4a4*>(x + min(c, lim - 4));  // uniform
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + 4 * (u * T + (int)threadIdx.x);
+            const int c = w0 + i;
+            if (i + 4 <= n && c + 4 <= cols) {
+                *reinterpret_cast<float4*>(xs + i) = make_float4(v[u].x, v[u].y, v[u].z, v[u].w);
+            } else if (i < n) {  // the end of the window or of x: element by element
+                const int d = c - min(c, lim - 4);  // shift of the clamped load
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float e = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (q == d + j) e = v[u][q];
+                    if (i + j < n) xs[i + j] = c + j < cols ? e : 0.f;
+                }
+            }
+        }
+    }
+}
+
 // Sum over all 64 lanes; every lane receives the total.
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll

@@ -95,8 +95,14 @@ struct DevBsr {
     const int* col;
     const float* data;
     int stride, brows, bh, bw, rows, cols, lanes;
+    const int2* win;      // sliced layout: per-workgroup x windows (LDS-windowed kernel), or null
 };
 constexpr int kBsrSlice = 64;  // block rows per slice of the sliced BSR layout
+// LDS-windowed sliced BSR: workgroups of kBsrWinRows block rows (one per
+// thread) stage x[w.x .. w.x + w.y) -- every column their blocks touch --
+// in LDS once, w.y <= kBsrWinFloats (160 KiB).
+constexpr int kBsrWinRows = 1024;
+constexpr int kBsrWinFloats = 40960;
 
 // Mixed-precision parts (k_mixed.hip, k_xtile.hip): value codecs, as CLSPMV_VAL_* in the ABI.
 constexpr int kCodecBf16x2 = 1;
@@ -154,6 +160,7 @@ struct DevBsrMixed {
     const void* data;     // bh*bw * stride encoded values
     const float* scale;   // FP16: one per block row; FP8: bh*bw * nsl
     int stride, brows, nsl, bh, bw, rows, cols, codec, delta;
+    const int2* win;      // per-workgroup x windows (LDS-windowed kernel), or null
 };
 
 // Batched multi-right-hand-side SpMV (k_spmm.hip).

@@ -197,6 +197,60 @@ const char* codec_name(int codec) {
                                     : "fp8slice";
 }
 
+// LDS-windowed sliced BSR kernels (tunable: "bsr_lds_window", 0 / 1).
+bool g_bsr_window = true;
+
+// Column repeated by the padding slots of an empty block row of the sliced
+// BSR layout: the first column of the slice's first nonempty block row
+// (inside the x window of the row's workgroup), else 0.
+std::vector<int> bsr_slice_fill(const clspmv_bsr_matrix* m, int brows) {
+    const int ns = (brows + kBsrSlice - 1) / kBsrSlice;
+    std::vector<int> fill((size_t)ns, 0);
+    for (int s = 0; s < ns; ++s)
+        for (int r = s * kBsrSlice; r < std::min(brows, s * kBsrSlice + kBsrSlice); ++r)
+            if (m->bsr_row_ptr[r] < m->bsr_row_ptr[r + 1]) {
+                fill[(size_t)s] = m->bsr_col_id[m->bsr_row_ptr[r]];
+                break;
+            }
+    return fill;
+}
+
+// x windows of the LDS-windowed sliced BSR kernels: for each workgroup of
+// kBsrWinRows block rows, {smallest slot column, largest slot column + bw -
+// smallest}.  Empty (the plain kernel is used) when the option is off, x
+// has fewer than 4 columns, some window holds more than kBsrWinFloats, or
+// the windows are narrow: a mean under 2 kBsrWinRows bw columns (twice the
+// rows' own width).  There the plain kernel's gathers hit few cache lines
+// per wave and its higher occupancy wins (1M 3x3 block rows, 9 blocks each
+// within +-b block columns: plain 71-74 us for b <= 200, 85 us at b = 1000,
+// 98 us at b = 5000; windowed 81 us throughout).
+std::vector<int2> bsr_windows(const std::vector<int>& sp, const std::vector<int>& scol, int brows, int bw,
+                              int cols) {
+    if (!g_bsr_window || brows == 0 || cols < 4) return {};
+    const int ns = (int)sp.size() - 1, per = kBsrWinRows / kBsrSlice;
+    const int nwg = (brows + kBsrWinRows - 1) / kBsrWinRows;
+    std::vector<int2> w((size_t)nwg);
+    long long total = 0;
+    for (int g = 0; g < nwg; ++g) {
+        int lo = INT32_MAX, hi = -1;
+        for (int s = g * per; s < std::min(ns, g * per + per); ++s)
+            for (int i = sp[(size_t)s]; i < sp[(size_t)s + 1]; ++i) {
+                lo = std::min(lo, scol[(size_t)i]);
+                hi = std::max(hi, scol[(size_t)i]);
+            }
+        if (hi < 0) {
+            w[(size_t)g] = make_int2(0, 0);
+            continue;
+        }
+        const long long len = (long long)hi + bw - lo;
+        if (len > kBsrWinFloats) return {};
+        w[(size_t)g] = make_int2(lo, (int)len);
+        total += len;
+    }
+    if (total < 2LL * kBsrWinRows * bw * nwg) return {};
+    return w;
+}
+
 int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
                      int cols);
 int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows, int cols,
@@ -1141,11 +1195,12 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
             const size_t sstride = std::max<size_t>(64, slots);
             std::vector<int> scol(slots, 0);
             std::vector<float> sdat(sstride * bs, 0.0f);
+            const std::vector<int> fill = bsr_slice_fill(m, brows);
             for (int r = 0; r < brows; ++r) {
                 const int s = r / S, t = r % S;
                 const int width = (sp[s + 1] - sp[s]) / S;
                 const int a = m->bsr_row_ptr[r], e = m->bsr_row_ptr[r + 1];
-                int last = 0;
+                int last = fill[(size_t)s];
                 for (int k = 0; k < width; ++k) {
                     const size_t idx = (size_t)sp[s] + (size_t)k * S + t;
                     if (a + k < e) {
@@ -1162,10 +1217,16 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
             if (rc) return rc;
             p->bsr.data = (const float*)p->add(ctx, sdat.data(), sdat.size() * 4, &rc);
             if (rc) return rc;
+            const std::vector<int2> win = bsr_windows(sp, scol, brows, (int)bw, cols);
+            const int2* dwin = nullptr;
+            if (!win.empty()) {
+                dwin = (const int2*)p->add(ctx, win.data(), win.size() * sizeof(int2), &rc);
+                if (rc) return rc;
+            }
             p->bsr = DevBsr{p->bsr.row_ptr, p->bsr.col, p->bsr.data, (int)sstride, brows, (int)bh, (int)bw,
-                            rows, cols, 0};
+                            rows, cols, 0, dwin};
             char name[32];
-            std::snprintf(name, sizeof(name), "bsr%u%u_sliced", bh, bw);
+            std::snprintf(name, sizeof(name), "bsr%u%u_sliced%s", bh, bw, dwin ? "_xw" : "");
             fill_info(p, CLSPMV_FMT_BSR, 0, rows, cols, m->matinfo.nnz, name, 1);
             return CLSPMV_OK;
         }
@@ -1294,10 +1355,11 @@ int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int code
         const size_t slots = (size_t)sp[ns], sstride = std::max<size_t>(64, slots);
         std::vector<int> scol(slots, 0), base((size_t)std::max(1, brows), 0);
         std::vector<float> sdat(sstride * bs, 0.0f);
+        const std::vector<int> fill = bsr_slice_fill(m, brows);
         for (int r = 0; r < brows; ++r) {
             const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
             const int a = m->bsr_row_ptr[r], e = m->bsr_row_ptr[r + 1];
-            int last = a < e ? m->bsr_col_id[a] : 0;
+            int last = a < e ? m->bsr_col_id[a] : fill[(size_t)s];
             base[(size_t)r] = last;
             for (int k = 0; k < width; ++k) {
                 const size_t idx = (size_t)sp[s] + (size_t)k * S + t;
@@ -1397,10 +1459,16 @@ int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int code
         d.cols = cols;
         d.codec = codec;
         d.delta = delta ? 1 : 0;
+        const std::vector<int2> win = bsr_windows(sp, scol, brows, (int)bw, cols);
+        if (!win.empty()) {
+            d.win = (const int2*)p->add(ctx, win.data(), win.size() * sizeof(int2), &rc);
+            if (rc) return rc;
+        }
         p->bsrm = d;
         p->mixed = true;
         char name[48];
-        std::snprintf(name, sizeof(name), "bsr%u%u_mixed_%s%s", bh, bw, codec_name(codec), delta ? "_d16" : "");
+        std::snprintf(name, sizeof(name), "bsr%u%u_mixed_%s%s%s", bh, bw, codec_name(codec), delta ? "_d16" : "",
+                      d.win ? "_xw" : "");
         fill_info(p, CLSPMV_FMT_BSR, kMethodMixed + codec + (delta ? 8 : 0), rows, cols, m->matinfo.nnz, name, 1);
         return CLSPMV_OK;
     });
@@ -1429,6 +1497,11 @@ int clspmv_set_option(const char* name, long long value) {
     if (std::strcmp(name, "xtile_max_groups") == 0) {
         if (value < 1 || value > kXtMaxGroups) return CLSPMV_E_INVALID;
         g_xt_max_groups = (int)value;
+        return CLSPMV_OK;
+    }
+    if (std::strcmp(name, "bsr_lds_window") == 0) {
+        if (value < 0 || value > 1) return CLSPMV_E_INVALID;
+        g_bsr_window = value != 0;
         return CLSPMV_OK;
     }
     if (std::strcmp(name, "xtile_blocks") == 0) {

@@ -236,6 +236,54 @@ def test_bsr_explicit_lanes(ctx, lanes):
     _check(coo, y, x, y0, f"BSR {info['kernel']}")
 
 
+def _bsr_window(flag):
+    L.check(L.lib().clspmv_set_option(b"bsr_lds_window", flag), "set_option")
+
+
+def _drop_rows(coo, lo, hi):
+    keep = (coo.row_id < lo) | (coo.row_id >= hi)
+    return M.CooMatrix.from_arrays(coo.row_id[keep].copy(), coo.col_id[keep].copy(), coo.data[keep].copy(),
+                                   coo.width, coo.height)
+
+
+# wide bands (x windows of 10k-30k columns per workgroup of 1024 block rows)
+BSR_WIN_CASES = [("band_5_workgroups", lambda: M.gen_block_rows(5000, 3, 9, 2000, 31)),
+                 ("ragged_uniform", lambda: rand_coo(3001, 20000, 40000, 14)),
+                 ("empty_block_rows", lambda: _drop_rows(M.gen_block_rows(6000, 3, 7, 2000, 32), 3000, 4700)),
+                 ("one_block_row", lambda: rand_coo(3, 30000, 40, 15))]
+
+
+@pytest.mark.parametrize("b", [2, 3, 4])
+@pytest.mark.parametrize("name,make", BSR_WIN_CASES, ids=[c[0] for c in BSR_WIN_CASES])
+def test_bsr_lds_window_matches_plain(ctx, b, name, make):
+    # the LDS-windowed sliced kernel does each lane's operations in the same
+    # order as the plain one: bitwise equal results, both at oracle parity
+    coo = make()
+    bsr = M.coo2bsr(coo, b, b)
+    x, y0 = _x_y(coo)
+    yw, iw = gpu_spmv(ctx, L.FMT_BSR, bsr, x, y0, 0)
+    _bsr_window(0)
+    try:
+        yp, ip = gpu_spmv(ctx, L.FMT_BSR, bsr, x, y0, 0)
+    finally:
+        _bsr_window(1)
+    assert iw["kernel"] == f"bsr{b}{b}_sliced_xw" and ip["kernel"] == f"bsr{b}{b}_sliced", (iw, ip)
+    assert np.array_equal(yw.view(np.uint32), yp.view(np.uint32)), name
+    _check(coo, yw, x, y0, f"BSR {name} {iw['kernel']}")
+
+
+@pytest.mark.parametrize("name,make", [("wider_than_lds", lambda: rand_coo(2000, 100000, 30000, 16)),
+                                       ("narrow_band", lambda: M.gen_block_rows(5000, 2, 9, 10, 17))])
+def test_bsr_window_out_of_range_uses_plain_kernel(ctx, name, make):
+    # columns spread over more x than a workgroup's LDS holds, or a band so
+    # narrow that the plain kernel's gathers are already local
+    coo = make()
+    x, y0 = _x_y(coo)
+    y, info = gpu_spmv(ctx, L.FMT_BSR, M.coo2bsr(coo, 2, 2), x, y0, 0)
+    assert info["kernel"] == "bsr22_sliced", info
+    _check(coo, y, x, y0, f"BSR {name} {info['kernel']}")
+
+
 def test_repeat_accumulates(ctx):
     # ntimes launches accumulate: y = y0 + 3 A x
     coo = M.gen_laplacian2d(16, 16)

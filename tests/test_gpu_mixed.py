@@ -113,6 +113,27 @@ def test_bsr_mixed(ctx, codec, b, name, make):
     p32.free()
 
 
+@pytest.mark.parametrize("codec", CODECS)
+def test_bsr_mixed_lds_window_matches_plain(ctx, codec):
+    # five workgroups of 1024 block rows, x windows staged in LDS, against
+    # the plain kernel (x read from global memory): bitwise equal
+    coo = M.gen_block_rows(5000, 3, 9, 2000, 33)
+    bsr = M.coo2bsr(coo, 3, 3)
+    x, y0 = M.gen_vector(coo.width, 3), M.gen_vector(coo.height, 4)
+    pw = M.MixedPart(ctx, L.FMT_BSR, bsr, codec, True)
+    L.check(L.lib().clspmv_set_option(b"bsr_lds_window", 0), "set_option")
+    try:
+        pp = M.MixedPart(ctx, L.FMT_BSR, bsr, codec, True)
+    finally:
+        L.check(L.lib().clspmv_set_option(b"bsr_lds_window", 1), "set_option")
+    assert pw.info["kernel"].endswith("_xw") and not pp.info["kernel"].endswith("_xw")
+    yw, yp = _run(ctx, pw, coo, x, y0), _run(ctx, pp, coo, x, y0)
+    assert np.array_equal(yw.view(np.uint32), yp.view(np.uint32))
+    _check(codec, yw, coo, x, y0, group=3)
+    pw.free()
+    pp.free()
+
+
 def test_bsr_delta_falls_back_on_wide_steps(ctx):
     # two blocks per block row 100000 columns apart: the step needs int32
     rows = np.array([0, 0, 1, 1], np.int32)
