@@ -175,8 +175,9 @@ __global__ __launch_bounds__(256) void bsr_kernel(const int* __restrict__ rp, co
 // load_xblock: no load sits behind a branch).  Requires cols >= BW.
 // XW: kBsrWinRows-thread workgroups first stage their x window win[] into
 // LDS (the columns of a wide band: one coalesced pass instead of a gather
-// of one cache line per lane and block); each lane's sums are the same
-// operations in the same order either way.
+// of one cache line per lane and block), with the column and value loads
+// of each lane's first G blocks issued ahead of the staging; each lane's
+// sums are the same operations in the same order either way.
 template <int BH, int BW, bool XW>
 __global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_sliced_kernel(
     const int* __restrict__ sp, const int* __restrict__ col, const float* __restrict__ data, int stride, int brows,
@@ -186,14 +187,7 @@ __global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_sliced_kernel(
     const int wg = xcd_tile(blockIdx.x, gridDim.x);
     const int r = wg * T + threadIdx.x;
     __shared__ __align__(16) float xs[XW ? kBsrWinFloats : 4];
-    int w0 = 0;
-    if constexpr (XW) {
-        const int2 w = win[wg];
-        w0 = w.x;
-        stage_xwindow<T>(x, w.x, w.y, cols, xs);
-        __syncthreads();
-    }
-    auto xblock = [&](int c, float(&xv)[BW]) {
+    auto xblock = [&](int w0, int c, float(&xv)[BW]) {
         if constexpr (XW) {
 #pragma unroll
             for (int j = 0; j < BW; ++j) xv[j] = xs[c - w0 + j];
@@ -201,21 +195,54 @@ __global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_sliced_kernel(
             load_xblock<BW>(x, c, cols, xv);
         }
     };
-    if (r >= brows) return;
+    auto load_group = [&](int b, int k, int t, int (&c)[G], float (&v)[G][BH * BW]) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) c[u] = ld_stream(col + b + ((k + u) << 6) + t);
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+#pragma unroll
+            for (int e = 0; e < BH * BW; ++e) v[u][e] = ld_stream(data + (size_t)e * stride + b + ((k + u) << 6) + t);
+    };
+    const bool live = r < brows;
+    if (!XW && !live) return;
     const int s = r >> 6, t = r & 63;
-    const int b = sp[s];
-    const int width = (sp[s + 1] - b) >> 6;
+    const int b = live ? sp[s] : 0;
+    const int width = live ? (sp[s + 1] - b) >> 6 : 0;
     float acc[BH];
 #pragma unroll
     for (int h = 0; h < BH; ++h) acc[h] = 0.f;
+    int w0 = 0;
     int k = 0;
+    if constexpr (XW) {
+        int c[G];
+        float v[G][BH * BW];
+        const bool first = width >= G;
+        if (first) load_group(b, 0, t, c, v);
+        const int2 w = win[wg];
+        w0 = w.x;
+        stage_xwindow<T>(x, w.x, w.y, cols, xs);
+        __syncthreads();
+        if (!live) return;
+        if (first) {
+            float xv[G][BW];
+#pragma unroll
+            for (int u = 0; u < G; ++u) xblock(w0, c[u], xv[u]);
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+#pragma unroll
+                for (int h = 0; h < BH; ++h)
+#pragma unroll
+                    for (int q = 0; q < BW; ++q) acc[h] = fmaf(v[u][h * BW + q], xv[u][q], acc[h]);
+            k = G;
+        }
+    }
     for (; k + G <= width; k += G) {
         int c[G];
 #pragma unroll
         for (int u = 0; u < G; ++u) c[u] = ld_stream(col + b + ((k + u) << 6) + t);
         float xv[G][BW], v[G][BH * BW];
 #pragma unroll
-        for (int u = 0; u < G; ++u) xblock(c[u], xv[u]);
+        for (int u = 0; u < G; ++u) xblock(w0, c[u], xv[u]);
 #pragma unroll
         for (int u = 0; u < G; ++u)
 #pragma unroll
@@ -230,7 +257,7 @@ __global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_sliced_kernel(
     for (; k < width; ++k) {
         const int idx = b + (k << 6) + t;
         float xv[BW];
-        xblock(ld_stream(col + idx), xv);
+        xblock(w0, ld_stream(col + idx), xv);
 #pragma unroll
         for (int h = 0; h < BH; ++h)
 #pragma unroll

@@ -190,9 +190,10 @@ __device__ __forceinline__ float bsr_value(const void* __restrict__ data, const 
 }
 
 // As bsr_sliced_kernel (blocks G at a time, loads issued back to back, x
-// through load_xblock or, XW, from the workgroup's x window staged in LDS;
-// cols >= BW), with encoded values and optionally int16 column steps: the
-// G steps are loaded together and summed in order.
+// through load_xblock or, XW, from the workgroup's x window staged in LDS
+// with the first G blocks' loads issued ahead of the staging; cols >= BW),
+// with encoded values and optionally int16 column steps: the G steps are
+// loaded together and summed in order.
 template <int BH, int BW, int CODEC, bool DELTA, bool XW>
 __global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_mixed_kernel(
     const int* __restrict__ sp, const void* __restrict__ colv, const int* __restrict__ base,
@@ -200,15 +201,10 @@ __global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_mixed_kernel(
     const int2* __restrict__ win, const float* __restrict__ x, float* __restrict__ y, int rows, int cols) {
     constexpr int G = bsr_group(BH * BW);
     constexpr int T = XW ? kBsrWinRows : 256;
-    const int r = xcd_tile(blockIdx.x, gridDim.x) * T + threadIdx.x;
+    const int wg = xcd_tile(blockIdx.x, gridDim.x);
+    const int r = wg * T + threadIdx.x;
     __shared__ __align__(16) float xs[XW ? kBsrWinFloats : 4];
     int w0 = 0;
-    if constexpr (XW) {
-        const int2 w = win[xcd_tile(blockIdx.x, gridDim.x)];
-        w0 = w.x;
-        stage_xwindow<T>(x, w.x, w.y, cols, xs);
-        __syncthreads();
-    }
     auto xblock = [&](int c, float(&xv)[BW]) {
         if constexpr (XW) {
 #pragma unroll
@@ -217,24 +213,32 @@ __global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_mixed_kernel(
             load_xblock<BW>(x, c, cols, xv);
         }
     };
-    if (r >= brows) return;
+    const bool live = r < brows;
+    if (!XW && !live) return;
     // a wave's 64 block rows are one slice: wave-uniform slice quantities
+    // (waves past the last block row have none and only stage x)
     const int s = __builtin_amdgcn_readfirstlane(r >> 6), t = r & 63;
-    const int b = sp[s];
-    const int width = (sp[s + 1] - b) >> 6;
+    const int b = live ? sp[s] : 0;
+    const int width = live ? (sp[s + 1] - b) >> 6 : 0;
     float acc[BH];
 #pragma unroll
     for (int h = 0; h < BH; ++h) acc[h] = 0.f;
-    int cprev = DELTA ? base[r] : 0;
+    int cprev = DELTA && live ? base[r] : 0;
     auto load_col = [&](int idx) {
         if (DELTA) return (int)ld_narrow(reinterpret_cast<const int16_t*>(colv) + idx);
         return ld_stream(reinterpret_cast<const int*>(colv) + idx);
     };
-    int k = 0;
-    for (; k + G <= width; k += G) {
-        int c[G];
+    // columns and values of blocks k .. k+G-1
+    auto load_group = [&](int k, int (&c)[G], float (&v)[G][BH * BW]) {
 #pragma unroll
         for (int u = 0; u < G; ++u) c[u] = load_col(b + ((k + u) << 6) + t);
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+#pragma unroll
+            for (int e = 0; e < BH * BW; ++e)
+                v[u][e] = bsr_value<CODEC>(data, scale, (size_t)e * stride + b + ((k + u) << 6) + t, e, nsl, s);
+    };
+    auto do_group = [&](int (&c)[G], float (&v)[G][BH * BW]) {
         if (DELTA) {
 #pragma unroll
             for (int u = 0; u < G; ++u) {
@@ -242,20 +246,37 @@ __global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_mixed_kernel(
                 c[u] = cprev;
             }
         }
-        float xv[G][BW], v[G][BH * BW];
+        float xv[G][BW];
 #pragma unroll
         for (int u = 0; u < G; ++u) xblock(c[u], xv[u]);
-#pragma unroll
-        for (int u = 0; u < G; ++u)
-#pragma unroll
-            for (int e = 0; e < BH * BW; ++e)
-                v[u][e] = bsr_value<CODEC>(data, scale, (size_t)e * stride + b + ((k + u) << 6) + t, e, nsl, s);
 #pragma unroll
         for (int u = 0; u < G; ++u)
 #pragma unroll
             for (int h = 0; h < BH; ++h)
 #pragma unroll
                 for (int w = 0; w < BW; ++w) acc[h] = fmaf(v[u][h * BW + w], xv[u][w], acc[h]);
+    };
+    int k = 0;
+    if constexpr (XW) {
+        int c[G];
+        float v[G][BH * BW];
+        const bool first = width >= G;
+        if (first) load_group(0, c, v);
+        const int2 w = win[wg];
+        w0 = w.x;
+        stage_xwindow<T>(x, w.x, w.y, cols, xs);
+        __syncthreads();
+        if (!live) return;
+        if (first) {
+            do_group(c, v);
+            k = G;
+        }
+    }
+    for (; k + G <= width; k += G) {
+        int c[G];
+        float v[G][BH * BW];
+        load_group(k, c, v);
+        do_group(c, v);
     }
     for (; k < width; ++k) {
         const int idx = b + (k << 6) + t;
