@@ -318,8 +318,9 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
  *   COO      0 segmented reduction, 2 column-sliced, 4 LDS x-tiled
  *   DIA      0 one row per lane, 1 four rows per lane
  *   BDIA     0 direct x reads, 1 LDS-staged x band
- *   BSR      0 sliced (64 block rows per slice), 1 (or L = 2..16) L lanes
- *            per block row
+ *   BSR      0 sliced (64 block rows per slice; with a wide band, x
+ *            windows of 1024 block rows staged in LDS, kernel name
+ *            suffix "_xw"), 1 (or L = 2..16) L lanes per block row
  * "Column-sliced" cuts the columns into slices whose part of x (<= 2 MiB by
  * default) stays in one XCD's L2; one launch per slice adds into y.
  * "LDS x-tiled" gives each 1024-thread workgroup a block of <= 32768 rows
@@ -359,6 +360,10 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  * created afterwards:
  *   "col_slice_bytes"   x bytes per column slice of the column-sliced
  *                       layouts (default 2 MiB);
+ *   "bsr_lds_window"    1 (default): sliced BSR parts (FP32 and mixed) stage
+ *                       each 1024-block-row workgroup's x window in LDS
+ *                       when every window fits (<= 40960 columns) and their
+ *                       mean exceeds 2048 bw columns; 0: never;
  *   "xtile_blocks"      target workgroups of the x-tiled layout (default
  *                       256, one per CU);
  *   "xtile_max_groups"  column groups per row block of the x-tiled layout
