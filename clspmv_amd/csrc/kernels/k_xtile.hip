@@ -21,7 +21,8 @@
 //     with a segmented shuffle scan; a segment that starts inside the chunk
 //     is added to its row by that wavefront alone, the chunk's first segment
 //     (which may continue an earlier chunk) goes to a per-chunk carry that is
-//     added after the next barrier, runs of equal rows in chunk order.
+//     added after the next tile's first barrier, runs of equal rows in chunk
+//     order (a tile's up to kXtRounds rounds of chunks run back to back).
 //   * x is staged once per row block (the x traffic is row blocks x x
 //     bytes), so row blocks are as tall as LDS allows; to still give every
 //     CU a workgroup, the slices are split into G <= 8 contiguous column
@@ -76,6 +77,16 @@ __device__ __forceinline__ void xt_apply_carry(const int* cr, const float* cv, f
     float s = y_l[r];
     for (int k = i; k < kXtWaves && cr[k] == r; ++k) s += cv[k];
     y_l[r] = s;
+}
+
+// The carry sets of the chunk rounds in `pending` (bit k: round k), in
+// round order, by lanes 0..kXtWaves-1 of wave 0 (in order: a row's carries
+// from two rounds are added one after the other).
+__device__ __forceinline__ void xt_apply_carries(int pending, int (*c_row)[kXtWaves], float (*c_val)[kXtWaves],
+                                                 float* y_l, int i) {
+#pragma unroll
+    for (int k = 0; k < kXtRounds; ++k)
+        if (pending & (1 << k)) xt_apply_carry(c_row[k], c_val[k], y_l, i);
 }
 
 // Adds one emitted segment sum to y_l -- unless it belongs to the chunk's
@@ -234,21 +245,24 @@ __device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int
 }
 
 struct XtState {
-    int par;
-    bool pending;
+    int pending;  // bit k: the carry set of chunk round k awaits its addition
 };
 
 // One tile: stage its x slice from set S, refill S with tile t + kXtDepth,
 // reduce its chunks.  Steps past the block's last tile (t >= t1) only keep
 // the barrier sequence and the carry bookkeeping, so that the unrolled loop
-// issues every load on every path.
+// issues every load on every path.  The chunk rounds of a tile need no
+// barrier between them: a row gets a direct addition from at most one
+// chunk of the tile (the one its entries start in, unless they start the
+// chunk), its other chunks' sums go to their carry slots, and each round
+// has its own carry set, added after the next tile's first barrier.
 template <int CODEC>
 __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, const XtArgs& a, int t, int t1, float* y_l, float4* x_l4,
                                         float* dummy, int (*c_row)[kXtWaves], float (*c_val)[kXtWaves], XtState& st,
                                         int tid, int wave, int lane) {
     __syncthreads();  // previous tile done with x_l; its carry slots are written
-    if (st.pending && tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
-    st.pending = false;
+    if (st.pending && tid < kXtWaves) xt_apply_carries(st.pending, c_row, c_val, y_l, tid);
+    st.pending = 0;
     const bool live = t < t1;
     const int tt = live ? t : t1 - 1;
     const int ts = a.tile_slice[tt];
@@ -278,17 +292,11 @@ __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, const XtArgs& a, int t,
     for (int k = 0; k < kXtRounds; ++k) {
         const int cb = c0 + k * kXtWaves;
         if (cb < c1) {
-            if (k > 0) {
-                __syncthreads();
-                if (tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
-            }
             if (cb + wave < c1)
-                xt_chunk(S.e[k], xt_decode<CODEC>(S.v[k], S.s[k]), x_l, y_l, dummy, c_row[st.par], c_val[st.par],
-                         wave, lane);
+                xt_chunk(S.e[k], xt_decode<CODEC>(S.v[k], S.s[k]), x_l, y_l, dummy, c_row[k], c_val[k], wave, lane);
             else if (lane == 0)
-                c_row[st.par][wave] = -1;
-            st.par ^= 1;
-            st.pending = true;
+                c_row[k][wave] = -1;
+            st.pending |= 1 << k;
         }
     }
     xt_fetch_e(S, a, tn, wave, lane);
@@ -305,8 +313,8 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     const XtArgs a{blk_row, blk_tile, tile_slice, tile_chunk, idx2, val, vscale, cols, x, y};
     __shared__ float y_l[kXtRows];
     __shared__ float4 x_l4[kXtX4];
-    __shared__ int c_row[2][kXtWaves];
-    __shared__ float c_val[2][kXtWaves];
+    __shared__ int c_row[kXtRounds][kXtWaves];
+    __shared__ float c_val[kXtRounds][kXtWaves];
     __shared__ float dummy[64];  // one per lane: where a lane's non-emits add 0
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = blockIdx.x;
@@ -329,14 +337,14 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
         xt_fetch_x(S[u], a, min(t0 + u, t1 - 1), tid);
         xt_fetch_e(S[u], a, min(t0 + u, t1 - 1), wave, lane);
     }
-    XtState st{0, false};
+    XtState st{0};
     for (int t = t0; t < t1; t += kXtDepth) {
 #pragma unroll
         for (int u = 0; u < kXtDepth; ++u)
             xt_step(S[u], a, t + u, t1, y_l, x_l4, dummy, c_row, c_val, st, tid, wave, lane);
     }
     __syncthreads();
-    if (st.pending && tid < kXtWaves) xt_apply_carry(c_row[st.par ^ 1], c_val[st.par ^ 1], y_l, tid);
+    if (st.pending && tid < kXtWaves) xt_apply_carries(st.pending, c_row, c_val, y_l, tid);
     __syncthreads();
     if constexpr (codec_row_scaled(CODEC))
         for (int r = tid; r < nr; r += kXtThreads) y_l[r] *= a.vscale[r0 + r];
