@@ -218,7 +218,9 @@ std::vector<int> bsr_slice_fill(const clspmv_bsr_matrix* m, int brows) {
 // x windows of the LDS-windowed sliced BSR kernels: for each workgroup of
 // kBsrWinRows block rows, {smallest slot column, largest slot column + bw -
 // smallest}.  Empty (the plain kernel is used) when the option is off, x
-// has fewer than 4 columns, some window holds more than kBsrWinFloats, or
+// has fewer than 4 columns (or so many that the staging's int column
+// arithmetic, window start + 4 kBsrWinFloats, could overflow), some window
+// holds more than kBsrWinFloats, or
 // the windows are narrow: a mean under 2 kBsrWinRows bw columns (twice the
 // rows' own width).  There the plain kernel's gathers hit few cache lines
 // per wave and its higher occupancy wins (1M 3x3 block rows, 9 blocks each
@@ -226,7 +228,7 @@ std::vector<int> bsr_slice_fill(const clspmv_bsr_matrix* m, int brows) {
 // 98 us at b = 5000; windowed 81 us throughout).
 std::vector<int2> bsr_windows(const std::vector<int>& sp, const std::vector<int>& scol, int brows, int bw,
                               int cols) {
-    if (!g_bsr_window || brows == 0 || cols < 4) return {};
+    if (!g_bsr_window || brows == 0 || cols < 4 || cols > INT32_MAX - 4 * kBsrWinFloats) return {};
     const int ns = (int)sp.size() - 1, per = kBsrWinRows / kBsrSlice;
     const int nwg = (brows + kBsrWinRows - 1) / kBsrWinRows;
     std::vector<int2> w((size_t)nwg);

@@ -204,6 +204,29 @@ def test_no_device_is_an_error_not_a_fallback():
     assert L.lib().clspmv_init(0, C.byref(h)) == L.E_NODEVICE
 
 
+def test_set_option_ranges():
+    # every documented option takes its documented values and refuses the
+    # rest (no device needed); the defaults are restored afterwards
+    setopt = L.lib().clspmv_set_option
+    cases = {b"col_slice_bytes": ([256, 1 << 21], [0, 255], 1 << 21),
+             b"xtile_blocks": ([1, 256, 1 << 20], [0, (1 << 20) + 1], 256),
+             b"xtile_max_groups": ([1, 8], [0, 9], 8),
+             b"bsr_lds_window": ([0, 1], [-1, 2], 1),
+             b"cocktail_value_codec": ([0, L.VAL_BF16X2, L.VAL_FP16_ROW, L.VAL_FP8_SLICE, L.VAL_I16_ROW],
+                                       [-1, L.VAL_I16_ROW + 1], 0)}
+    try:
+        for name, (good, bad, _) in cases.items():
+            for v in good:
+                assert setopt(name, v) == L.OK, (name, v)
+            for v in bad:
+                assert setopt(name, v) == L.E_INVALID, (name, v)
+        assert setopt(b"no_such_option", 1) == L.E_INVALID
+        assert setopt(None, 1) == L.E_INVALID
+    finally:
+        for name, (_, _, default) in cases.items():
+            assert setopt(name, default) == L.OK
+
+
 # ------------------------------------------------------------- failure paths
 def test_dia_refuses_too_much_padding():
     coo = M.CooMatrix.from_arrays(np.arange(100, dtype=np.int32), (np.arange(100) * 37 % 100).astype(np.int32),
