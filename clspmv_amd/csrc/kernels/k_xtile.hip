@@ -101,6 +101,12 @@ __device__ __forceinline__ void xt_add(bool on, int row, float v, int first_row,
     *p += v;
 }
 
+// Lane l receives v of lane l - 1 (lane 0 keeps its own): the DPP
+// shift-the-whole-wave-right-by-one mode (control 0x138) of GFX9-family
+// parts, a VALU move instead of a trip through the LDS crossbar.
+__device__ __forceinline__ int lane_up1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_up1(float v) { return __int_as_float(lane_up1(__float_as_int(v))); }
+
 // One wavefront reduces one 128-entry chunk (2 entries per lane).  Adds the
 // sums of segments that start inside the chunk to y_l and writes the
 // chunk's carry slot for its first row.  Each segment is emitted once, by
@@ -118,10 +124,10 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
     const float head_sum = pa;
     const int tail_row = rb;
     const float tail_sum = brk ? pb : pa + pb;
-    const int first_row = __shfl(head_row, 0, 64);
+    const int first_row = __builtin_amdgcn_readfirstlane(head_row);  // every lane is active here
 
-    const int prev_tail_row = __shfl_up(tail_row, 1, 64);
-    const float prev_tail_sum = __shfl_up(tail_sum, 1, 64);
+    const int prev_tail_row = lane_up1(tail_row);
+    const float prev_tail_sum = lane_up1(tail_sum);
     const bool cont = lane > 0 && prev_tail_row == head_row;  // head continues the previous lane's tail
     const bool through = cont && !brk;                        // the whole lane is inside that segment
     const unsigned long long tmask = __ballot(through);
@@ -152,7 +158,7 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
             cval = take ? cval + up_v : cval;
             flag = take ? up_f : flag;
         }
-        const float prev_c = __shfl_up(cval, 1, 64);
+        const float prev_c = lane_up1(cval);
         // a segment that ended exactly before this lane
         const bool boundary_end = lane > 0 && prev_tail_row != head_row;
         on1 = boundary_end, row1 = prev_tail_row, s1 = prev_c;
