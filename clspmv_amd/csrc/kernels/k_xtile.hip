@@ -107,6 +107,23 @@ __device__ __forceinline__ void xt_add(bool on, int row, float v, int first_row,
 __device__ __forceinline__ int lane_up1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
 __device__ __forceinline__ float lane_up1(float v) { return __int_as_float(lane_up1(__float_as_int(v))); }
 
+// v of another lane by a DPP mode of GFX9-family parts: CTRL 0x110 + d is
+// "the lane d below in the same 16-lane row", 0x142 / 0x143 broadcast lane
+// 15 / 31 into the next row(s); ROWS selects the rows written (the others,
+// and lanes with no source, read 0).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false); }
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f(float v) { return __int_as_float(dpp_i<CTRL, ROWS>(__float_as_int(v))); }
+
+// One step of a segmented inclusive scan: a lane with no segment start in
+// its range so far takes the range below (value added, start flag taken).
+__device__ __forceinline__ void seg_step(bool valid, float up_v, int up_f, float& v, int& f) {
+    const bool take = valid && !f;
+    v = take ? v + up_v : v;
+    f = take ? up_f : f;
+}
+
 // One wavefront reduces one 128-entry chunk (2 entries per lane).  Adds the
 // sums of segments that start inside the chunk to y_l and writes the
 // chunk's carry slot for its first row.  Each segment is emitted once, by
@@ -116,6 +133,9 @@ __device__ __forceinline__ float lane_up1(float v) { return __int_as_float(lane_
 // with a ballot and a readlane.
 __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, float* y_l, float* dummy, int* cr,
                                          float* cv, int wave, int lane) {
+    // products rounded, then added: no multiply-add contraction, so that
+    // every value codec (and every compilation) sums the same FP32 numbers
+#pragma clang fp contract(off)
     const int ra = (int)(e.x >> 16), rb = (int)(e.y >> 16);
     const float pa = v.x * x_l[e.x & 0xFFFFu];
     const float pb = v.y * x_l[e.y & 0xFFFFu];
@@ -147,17 +167,19 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
         xt_add(on1, row1, s1, first_row, y_l, dummy, lane);
         xt_add(on2, row2, s2, first_row, y_l, dummy, lane);
     } else {
-        // General case: segmented inclusive scan of the lane tails.
+        // General case: segmented inclusive scan of the lane tails, all in
+        // DPP moves: steps of 1, 2, 4, 8 lanes within each 16-lane row, then
+        // row 0's prefix into rows 1 and 3 (lane 15 of the row below) and
+        // rows 0-1's into rows 2 and 3 (lane 31).
         int flag = (lane == 0) || brk || (prev_tail_row != tail_row);
         float cval = tail_sum;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const float up_v = __shfl_up(cval, d, 64);
-            const int up_f = __shfl_up(flag, d, 64);
-            const bool take = lane >= d && !flag;
-            cval = take ? cval + up_v : cval;
-            flag = take ? up_f : flag;
-        }
+        const int rl = lane & 15;
+        seg_step(rl >= 1, dpp_f<0x111, 0xf>(cval), dpp_i<0x111, 0xf>(flag), cval, flag);
+        seg_step(rl >= 2, dpp_f<0x112, 0xf>(cval), dpp_i<0x112, 0xf>(flag), cval, flag);
+        seg_step(rl >= 4, dpp_f<0x114, 0xf>(cval), dpp_i<0x114, 0xf>(flag), cval, flag);
+        seg_step(rl >= 8, dpp_f<0x118, 0xf>(cval), dpp_i<0x118, 0xf>(flag), cval, flag);
+        seg_step((lane & 16) != 0, dpp_f<0x142, 0xa>(cval), dpp_i<0x142, 0xa>(flag), cval, flag);
+        seg_step(lane >= 32, dpp_f<0x143, 0xc>(cval), dpp_i<0x143, 0xc>(flag), cval, flag);
         const float prev_c = lane_up1(cval);
         // a segment that ended exactly before this lane
         const bool boundary_end = lane > 0 && prev_tail_row != head_row;

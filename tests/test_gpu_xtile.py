@@ -46,8 +46,24 @@ def _rand_wide(h, w, nnz, seed):
     return M.CooMatrix.from_arrays(r, c, rng.standard_normal(r.size).astype(np.float32), w, h)
 
 
+def _run_lengths():
+    """Rows of 1..97 entries (lengths (7 r) mod 97 + 1) in one column slice:
+    segments of every length start at every lane of a chunk, crossing the
+    16-lane rows of the wave (the chunk reduction's general path)."""
+    rng = np.random.default_rng(16)
+    rows, cols = [], []
+    for r in range(700):
+        n = (7 * r) % 97 + 1
+        rows.append(np.full(n, r, np.int32))
+        cols.append(np.sort(rng.choice(4000, n, replace=False)).astype(np.int32))
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    return M.CooMatrix.from_arrays(r, c, rng.standard_normal(r.size).astype(np.float32), 4000, 700)
+
+
 CASES = [
     ("lap2d_32", lambda: M.gen_laplacian2d(32, 32)),
+    ("run_lengths", _run_lengths),
     ("uniform_rows", lambda: M.gen_uniform_rows(3000, 17, 3)),
     ("zipf", lambda: M.gen_zipf_rows(5000, 1.8, 16.0, 3000, 11)),
     ("rand_ragged", lambda: rand_coo(333, 301, 4000, 1)),
