@@ -47,6 +47,7 @@ namespace clspmv {
 namespace dev {
 
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
@@ -266,8 +267,16 @@ __device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int
         // a wave with no chunk in this chunk round reloads the tile's last chunk
         // (a cache hit) rather than fetching the next tile's entries twice
         const int c = __builtin_amdgcn_readfirstlane(min(c0 + wave + k * kXtWaves, c1 - 1));
-        S.e[k] = __builtin_nontemporal_load(a.idx2 + (size_t)c * 64 + lane);
-        S.v[k] = __builtin_nontemporal_load(reinterpret_cast<const V*>(a.val) + (size_t)c * 64 + lane);
+        if constexpr (CODEC == 0) {
+            // FP32: a lane's two entries and two values interleaved, one
+            // 16-byte load (half the load instructions of two streams)
+            const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a.idx2) + (size_t)c * 64 + lane);
+            S.e[k] = u32x2{q.x, q.y};
+            S.v[k] = f32x2{__uint_as_float(q.z), __uint_as_float(q.w)};
+        } else {
+            S.e[k] = __builtin_nontemporal_load(a.idx2 + (size_t)c * 64 + lane);
+            S.v[k] = __builtin_nontemporal_load(reinterpret_cast<const V*>(a.val) + (size_t)c * 64 + lane);
+        }
         if constexpr (CODEC == kCodecFp8Slice) S.s[k] = a.vscale[c];
     }
 }

@@ -785,8 +785,25 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
     if (rc) return rc;
     d.tile_chunk = (const int*)p->add(ctx, tile_chunk.data(), tile_chunk.size() * 4, &rc);
     if (rc) return rc;
-    d.idx = (const unsigned*)p->add(ctx, idx.data(), idx.size() * 4, &rc);
-    if (rc) return rc;
+    if (codec == 0) {
+        // FP32: per chunk and lane {entry 2l, entry 2l+1, value 2l, value
+        // 2l+1}, one 16-byte load per lane (val is not used)
+        const size_t nl = (size_t)tile_chunk.back() * 64;
+        std::vector<uint32_t> ev(std::max<size_t>(nl, 1) * 4, 0);
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+        for (long long i = 0; i < (long long)nl; ++i) {
+            uint32_t* o = ev.data() + (size_t)i * 4;
+            o[0] = idx[(size_t)i * 2];
+            o[1] = idx[(size_t)i * 2 + 1];
+            std::memcpy(o + 2, &val[(size_t)i * 2], 8);
+        }
+        d.idx = (const unsigned*)p->add(ctx, ev.data(), ev.size() * 4, &rc);
+        if (rc) return rc;
+        d.val = d.idx;
+    } else {
+        d.idx = (const unsigned*)p->add(ctx, idx.data(), idx.size() * 4, &rc);
+        if (rc) return rc;
+    }
     std::vector<float> vscale;
     if (codec == kCodecBf16x2) {
         std::vector<uint32_t> w(val.size());
@@ -813,8 +830,6 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
             for (int i = 0; i < kXtChunk; ++i) q[(size_t)c * kXtChunk + (size_t)i] = f32_to_e4m3(v[i] / sc);
         }
         d.val = p->add(ctx, q.data(), q.size(), &rc);
-    } else {
-        d.val = p->add(ctx, val.data(), val.size() * 4, &rc);
     }
     if (rc) return rc;
     if (!vscale.empty()) {
