@@ -43,12 +43,13 @@ constexpr int kXtile = 4;  // method number of the x-tiled layout
 // Layout choice for the row-list parts (ELL, SELL, CSR, COO) when x is wider
 // than one XCD's L2 (4 MiB): the x-tiled layout (method 4) when its cost is
 // below the gather cost, else the column-sliced layout (method 2).  Costs
-// fitted on MI355X: gather ~ 26 us per 2 MiB column slice + 4.65 ps per
-// entry (uniform random columns, 4M x 4M); x-tiled ~ the larger of its x
-// staging (every row block -- at most 32768 rows, at least 32 blocks --
-// stages each 8064-column slice it touches, at ~4.1 TB/s) and 3.9 ps per
-// entry (4M x 32 uniform: 520 us), plus the pass adding the column groups'
-// partial sums.
+// fitted on MI355X (uniform random columns, 1M and 4M rows and columns,
+// 1-32 entries per row; Zipf rows): gather ~ 6.2 ps per row and 2 MiB
+// column slice + 4.65 ps per entry (4M x 4M x 1: 213 us); x-tiled ~ the
+// larger of its x staging (every row block -- at most 32768 rows, at least
+// 32 blocks -- stages each 8064-column slice it touches, at ~6.7 TB/s:
+// 4M x 4M, 1-16 per row: 323-360 us) and 3 ps per entry, plus the pass
+// adding the column groups' partial sums and ~10 us of start-up.
 bool wide_x(int cols) { return 4.0 * cols > 4.0 * 1024 * 1024; }
 bool xtile_pays(long long rows, long long cols, long long nnz) {
     if (nnz <= 0 || cols < 4) return false;
@@ -57,8 +58,9 @@ bool xtile_pays(long long rows, long long cols, long long nnz) {
     const double groups = std::max(1.0, std::min(8.0, std::floor(256.0 / nb)));
     const double per_block = nnz / nb;
     const double staged = nb * ns * (1.0 - std::exp(-per_block / ns)) * 32256.0;
-    const double t_xt = std::max(staged / 4.1e12, 3.9e-12 * nnz) + (groups > 1 ? (groups + 2) * rows * 4.0 / 5e12 : 0);
-    const double t_gather = 26e-6 * std::ceil(4.0 * cols / (2.0 * 1024 * 1024)) + 4.65e-12 * nnz;
+    const double t_xt = std::max(staged / 6.7e12, 3.0e-12 * nnz) +
+                        (groups > 1 ? (groups + 2) * rows * 4.0 / 5e12 : 0) + 10e-6;
+    const double t_gather = 6.2e-12 * rows * std::ceil(4.0 * cols / (2.0 * 1024 * 1024)) + 4.65e-12 * nnz;
     return t_xt < t_gather;
 }
 
