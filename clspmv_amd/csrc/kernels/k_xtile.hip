@@ -69,26 +69,6 @@ __device__ __forceinline__ float4 x_store4(f32x4u v, int c, int cols) {
     return o;
 }
 
-// Adds the run of equal-row carries that starts at slot i (slots in chunk
-// order; row -1 marks an unused slot).
-__device__ __forceinline__ void xt_apply_carry(const int* cr, const float* cv, float* y_l, int i) {
-    const int r = cr[i];
-    if (r < 0) return;
-    if (i > 0 && cr[i - 1] == r) return;
-    float s = y_l[r];
-    for (int k = i; k < kXtWaves && cr[k] == r; ++k) s += cv[k];
-    y_l[r] = s;
-}
-
-// The carry sets of the chunk rounds in `pending` (bit k: round k), in
-// round order, by lanes 0..kXtWaves-1 of wave 0 (in order: a row's carries
-// from two rounds are added one after the other).
-__device__ __forceinline__ void xt_apply_carries(int pending, int (*c_row)[kXtWaves], float (*c_val)[kXtWaves],
-                                                 float* y_l, int i) {
-#pragma unroll
-    for (int k = 0; k < kXtRounds; ++k)
-        if (pending & (1 << k)) xt_apply_carry(c_row[k], c_val[k], y_l, i);
-}
 
 // Adds one emitted segment sum to y_l -- unless it belongs to the chunk's
 // first row (that one goes to the chunk's carry slot) or to padding -- with
@@ -123,6 +103,33 @@ __device__ __forceinline__ void seg_step(bool valid, float up_v, int up_f, float
     const bool take = valid && !f;
     v = take ? v + up_v : v;
     f = take ? up_f : f;
+}
+
+static_assert(kXtRounds * kXtWaves == 32, "the carry slots of a tile fill lanes 0..31 of one wave");
+
+// Adds the carry sets of the chunk rounds in `pending` (bit k: round k) to
+// y_l, by one whole wave: lane 16 k + i holds slot i of round k (chunk
+// order; row -1 marks an unused slot, as do rounds not in `pending`), runs
+// of equal rows are summed by a segmented scan in DPP moves (row shifts,
+// then lane 15 into lanes 16..31), and the last lane of each run adds the
+// run's sum to its row -- one LDS read-modify-write per row.
+__device__ __forceinline__ void xt_apply_carries(int pending, int (*c_row)[kXtWaves], float (*c_val)[kXtWaves],
+                                                 float* y_l, int lane) {
+    const int k = (lane >> 4) & 1, i = lane & 15;  // lanes 32..63 mirror 0..31 (unused)
+    const bool use = lane < 32 && ((pending >> k) & 1);
+    const int r = use ? c_row[k][i] : -1;
+    float v = use ? c_val[k][i] : 0.f;
+    const int r_prev = lane_up1(r);
+    int flag = lane == 0 || r != r_prev;
+    const int rl = lane & 15;
+    seg_step(rl >= 1, dpp_f<0x111, 0xf>(v), dpp_i<0x111, 0xf>(flag), v, flag);
+    seg_step(rl >= 2, dpp_f<0x112, 0xf>(v), dpp_i<0x112, 0xf>(flag), v, flag);
+    seg_step(rl >= 4, dpp_f<0x114, 0xf>(v), dpp_i<0x114, 0xf>(flag), v, flag);
+    seg_step(rl >= 8, dpp_f<0x118, 0xf>(v), dpp_i<0x118, 0xf>(flag), v, flag);
+    seg_step((lane & 16) != 0, dpp_f<0x142, 0xa>(v), dpp_i<0x142, 0xa>(flag), v, flag);
+    // the run's last slot: the next lane holds another row (lanes 31.. end)
+    const int r_next = __builtin_amdgcn_update_dpp(-1, r, 0x130, 0xf, 0xf, false);  // lane + 1 (wave shift left by one)
+    if (lane < 32 && r >= 0 && (lane == 31 || r_next != r)) y_l[r] += v;
 }
 
 // One wavefront reduces one 128-entry chunk (2 entries per lane).  Adds the
@@ -298,7 +305,7 @@ __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, const XtArgs& a, int t,
                                         float* dummy, int (*c_row)[kXtWaves], float (*c_val)[kXtWaves], XtState& st,
                                         int tid, int wave, int lane) {
     __syncthreads();  // previous tile done with x_l; its carry slots are written
-    if (st.pending && tid < kXtWaves) xt_apply_carries(st.pending, c_row, c_val, y_l, tid);
+    if (st.pending && wave == 0) xt_apply_carries(st.pending, c_row, c_val, y_l, lane);
     st.pending = 0;
     const bool live = t < t1;
     const int tt = live ? t : t1 - 1;
@@ -381,7 +388,7 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
             xt_step(S[u], a, t + u, t1, y_l, x_l4, dummy, c_row, c_val, st, tid, wave, lane);
     }
     __syncthreads();
-    if (st.pending && tid < kXtWaves) xt_apply_carries(st.pending, c_row, c_val, y_l, tid);
+    if (st.pending && wave == 0) xt_apply_carries(st.pending, c_row, c_val, y_l, lane);
     __syncthreads();
     if constexpr (codec_row_scaled(CODEC))
         for (int r = tid; r < nr; r += kXtThreads) y_l[r] *= a.vscale[r0 + r];
