@@ -16,13 +16,16 @@
 //     which covers the HBM/MALL latency that a one-tile prefetch leaves
 //     exposed;
 //   * a tile's entries are sorted by (row, col), packed as
-//     (row_in_block << 16 | col_in_slice) plus the value, and padded to
-//     128-entry chunks.  A wavefront reduces one chunk (2 entries per lane)
-//     with a segmented shuffle scan; a segment that starts inside the chunk
-//     is added to its row by that wavefront alone, the chunk's first segment
-//     (which may continue an earlier chunk) goes to a per-chunk carry that is
-//     added after the next tile's first barrier, runs of equal rows in chunk
-//     order (a tile's up to kXtRounds rounds of chunks run back to back).
+//     (row_in_block << 16 | col_in_slice) plus the value (FP32: a lane's two
+//     entries and two values in one 16-byte word), and padded to 128-entry
+//     chunks.  A wavefront reduces one chunk (2 entries per lane) with a
+//     segmented scan whose lane exchanges are all DPP moves (no trip through
+//     the LDS crossbar); a segment that starts inside the chunk is added to
+//     its row by that wavefront alone, the chunk's first segment (which may
+//     continue an earlier chunk) goes to a per-chunk carry slot.  A tile's
+//     up to kXtRounds rounds of chunks run back to back; after the next
+//     tile's first barrier one wave sums the carry slots of all its rounds
+//     per run of equal rows (another DPP scan) and adds each run once.
 //   * x is staged once per row block (the x traffic is row blocks x x
 //     bytes), so row blocks are as tall as LDS allows; to still give every
 //     CU a workgroup, the slices are split into G <= 8 contiguous column
@@ -36,9 +39,9 @@
 //
 // Mixed-precision values (the CODEC template argument; FP32 products and
 // sums): BF16 hi + lo words (4 bytes per entry, as FP32), FP16 divided by a
-// power-of-two row scale that is applied once per row when the block's sums
-// leave LDS (2 bytes), or FP8 E4M3 times one FP32 scale per 128-entry chunk
-// (1 byte).
+// power-of-two row scale or int16 fixed point over a row scale, the scale
+// applied once per row when the block's sums leave LDS (2 bytes), or FP8
+// E4M3 times one FP32 scale per 128-entry chunk (1 byte).
 #include "kcodec.hpp"
 #include "kcommon.hpp"
 #include "launch.hpp"
