@@ -183,6 +183,7 @@ __global__ __launch_bounds__(256) void dia_mixed_kernel(const int* __restrict__ 
 template <int CODEC>
 __device__ __forceinline__ float bsr_value(const void* __restrict__ data, const float* __restrict__ scale, size_t i,
                                           int e, int nsl, int s) {
+    if (CODEC == 0) return ld_stream(reinterpret_cast<const float*>(data) + i);  // plain FP32
     if (CODEC == kCodecBf16x2) return bf16x2_value(ld_stream(reinterpret_cast<const uint32_t*>(data) + i));
     if (CODEC == kCodecFp16Row) return f16_value(ld_narrow(reinterpret_cast<const uint16_t*>(data) + i));
     if (CODEC == kCodecI16Row) return (float)ld_narrow(reinterpret_cast<const int16_t*>(data) + i);
@@ -329,12 +330,14 @@ static void bsr_mixed_launch(const DevBsrMixed& m, const float* x, float* y, hip
     hipLaunchKernelGGL((dev::bsr_mixed_kernel<BH, BW, C, D, XW>), g, b, 0, st, m.sp, m.col, m.base, m.data,       \
                        m.scale, m.stride, m.brows, m.nsl, m.win, x, y, m.rows, m.cols)
     if (m.delta) {
-        if (m.codec == kCodecBf16x2) CLSPMV_BSR_MIXED(kCodecBf16x2, true);
+        if (m.codec == 0) CLSPMV_BSR_MIXED(0, true);
+        else if (m.codec == kCodecBf16x2) CLSPMV_BSR_MIXED(kCodecBf16x2, true);
         else if (m.codec == kCodecFp16Row) CLSPMV_BSR_MIXED(kCodecFp16Row, true);
         else if (m.codec == kCodecI16Row) CLSPMV_BSR_MIXED(kCodecI16Row, true);
         else CLSPMV_BSR_MIXED(kCodecFp8Slice, true);
     } else {
-        if (m.codec == kCodecBf16x2) CLSPMV_BSR_MIXED(kCodecBf16x2, false);
+        if (m.codec == 0) CLSPMV_BSR_MIXED(0, false);
+        else if (m.codec == kCodecBf16x2) CLSPMV_BSR_MIXED(kCodecBf16x2, false);
         else if (m.codec == kCodecFp16Row) CLSPMV_BSR_MIXED(kCodecFp16Row, false);
         else if (m.codec == kCodecI16Row) CLSPMV_BSR_MIXED(kCodecI16Row, false);
         else CLSPMV_BSR_MIXED(kCodecFp8Slice, false);

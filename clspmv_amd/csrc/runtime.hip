@@ -257,6 +257,7 @@ int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int
                      int cols);
 int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows, int cols,
                 int codec = 0);
+int build_bsr_coded(clspmv_ctx_t ctx, clspmv_part_t p, const clspmv_bsr_matrix* m, int codec, int delta_cols);
 
 // Row-list view (CSR arrays) of a stored format, for the layouts that are
 // rebuilt from rows (column-sliced, x-tiled).  Every stored slot is kept,
@@ -852,6 +853,168 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
     return CLSPMV_OK;
 }
 
+// Sliced BSR r x r (2 <= r <= 4) with values in `codec` (0: FP32 as stored)
+// and, with delta_cols, block columns as int16 steps within a block row when
+// every step fits (else int32).  Codec 0 is the FP32 sliced part of
+// clspmv_part_bsr method 0 (kernel name bsr<r><r>_sliced[_d16][_xw]).
+int build_bsr_coded(clspmv_ctx_t ctx, clspmv_part_t p, const clspmv_bsr_matrix* m, int codec, int delta_cols) {
+    {
+        const unsigned bw = m->bsr_bwidth, bh = m->bsr_bheight;
+        if (bw != bh || bw < 2 || bw > 4) return CLSPMV_E_UNSUPPORTED;
+        const int rows = m->matinfo.height, cols = m->matinfo.width;
+        if (cols < (int)bw) {
+            set_error("mixed BSR needs at least bw columns");
+            return CLSPMV_E_UNSUPPORTED;
+        }
+        const int brows = m->bsr_row_num, nb = m->bsr_block_num, stride = m->bsr_block_stride;
+        if (brows != (rows + (int)bh - 1) / (int)bh || stride < nb || m->bsr_row_ptr[0] != 0 ||
+            m->bsr_row_ptr[brows] != nb)
+            return CLSPMV_E_INVALID;
+        for (int r = 0; r < brows; ++r)
+            if (m->bsr_row_ptr[r + 1] < m->bsr_row_ptr[r]) return CLSPMV_E_INVALID;
+        if (!cols_ok(m->bsr_col_id, (size_t)nb, 0, std::max(1, cols))) {
+            set_error("BSR block column out of range");
+            return CLSPMV_E_INVALID;
+        }
+        const unsigned bs = bw * bh;
+        const int S = kBsrSlice, ns = (brows + S - 1) / S;
+        std::vector<int> sp((size_t)ns + 1, 0);
+        for (int s = 0; s < ns; ++s) {
+            int wmax = 0;
+            for (int r = s * S; r < std::min(brows, s * S + S); ++r)
+                wmax = std::max(wmax, m->bsr_row_ptr[r + 1] - m->bsr_row_ptr[r]);
+            if ((int64_t)sp[s] + (int64_t)wmax * S > INT32_MAX) return CLSPMV_E_TOO_LARGE;
+            sp[s + 1] = sp[s] + wmax * S;
+        }
+        const size_t slots = (size_t)sp[ns], sstride = std::max<size_t>(64, slots);
+        std::vector<int> scol(slots, 0), base((size_t)std::max(1, brows), 0);
+        std::vector<float> sdat(sstride * bs, 0.0f);
+        const std::vector<int> fill = bsr_slice_fill(m, brows);
+        for (int r = 0; r < brows; ++r) {
+            const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
+            const int a = m->bsr_row_ptr[r], e = m->bsr_row_ptr[r + 1];
+            int last = a < e ? m->bsr_col_id[a] : fill[(size_t)s];
+            base[(size_t)r] = last;
+            for (int k = 0; k < width; ++k) {
+                const size_t idx = (size_t)sp[s] + (size_t)k * S + t;
+                if (a + k < e) {
+                    last = m->bsr_col_id[a + k];
+                    for (unsigned q = 0; q < bs; ++q) sdat[(size_t)q * sstride + idx] = m->bsr_data[(size_t)q * stride + a + k];
+                }
+                scol[idx] = last;
+            }
+        }
+        // int16 deltas when every step fits, else int32 columns
+        bool delta = delta_cols != 0;
+        std::vector<int16_t> dcol;
+        if (delta) {
+            dcol.assign(slots, 0);
+            for (int r = 0; r < brows && delta; ++r) {
+                const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
+                int prev = base[(size_t)r];
+                for (int k = 0; k < width; ++k) {
+                    const size_t idx = (size_t)sp[s] + (size_t)k * S + t;
+                    const int dlt = scol[idx] - prev;
+                    if (dlt < INT16_MIN || dlt > INT16_MAX) {
+                        delta = false;
+                        break;
+                    }
+                    dcol[idx] = (int16_t)dlt;
+                    prev = scol[idx];
+                }
+            }
+        }
+        const int nsl = ns;
+        std::vector<float> scale;
+        std::vector<unsigned char> enc;
+        const size_t nv = sstride * bs;
+        if (codec == 0) {
+            enc.assign(reinterpret_cast<unsigned char*>(sdat.data()), reinterpret_cast<unsigned char*>(sdat.data()) + nv * 4);
+        } else if (codec == kCodecBf16x2) {
+            std::vector<uint32_t> w(nv);
+            for (size_t i = 0; i < nv; ++i) w[i] = bf16x2_word(sdat[i]);
+            enc.assign(reinterpret_cast<unsigned char*>(w.data()), reinterpret_cast<unsigned char*>(w.data()) + nv * 4);
+        } else if (codec_row_scaled(codec)) {
+            scale.assign((size_t)std::max(1, brows), 1.0f);
+            for (int r = 0; r < brows; ++r) {
+                const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
+                float amax = 0.f;
+                for (int k = 0; k < width; ++k)
+                    for (unsigned q = 0; q < bs; ++q)
+                        amax = std::max(amax, std::fabs(sdat[(size_t)q * sstride + (size_t)sp[s] + (size_t)k * S + t]));
+                scale[(size_t)r] = row_scale(codec, amax);
+            }
+            if (int e = check_row_scales(codec, scale)) return e;
+            std::vector<uint16_t> h(nv, 0);
+            for (int r = 0; r < brows; ++r) {
+                const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
+                for (int k = 0; k < width; ++k)
+                    for (unsigned q = 0; q < bs; ++q) {
+                        const size_t i = (size_t)q * sstride + (size_t)sp[s] + (size_t)k * S + t;
+                        h[i] = row_encode(codec, sdat[i] / scale[(size_t)r]);
+                    }
+            }
+            enc.assign(reinterpret_cast<unsigned char*>(h.data()), reinterpret_cast<unsigned char*>(h.data()) + nv * 2);
+        } else {
+            scale.assign((size_t)bs * nsl, 1.0f);
+            enc.assign(nv, 0);
+            for (unsigned q = 0; q < bs; ++q)
+                for (int s = 0; s < ns; ++s) {
+                    float amax = 0.f;
+                    for (int i = sp[s]; i < sp[s + 1]; ++i) amax = std::max(amax, std::fabs(sdat[(size_t)q * sstride + i]));
+                    const float sc = fp8_scale(amax);
+                    scale[(size_t)q * nsl + s] = sc;
+                    for (int i = sp[s]; i < sp[s + 1]; ++i)
+                        enc[(size_t)q * sstride + i] = f32_to_e4m3(sdat[(size_t)q * sstride + i] / sc);
+                }
+        }
+        int rc = 0;
+        DevBsrMixed d{};
+        d.sp = (const int*)p->add(ctx, sp.data(), sp.size() * 4, &rc);
+        if (rc) return rc;
+        if (delta) {
+            d.col = p->add(ctx, dcol.data(), std::max<size_t>(16, slots * 2), &rc);
+            if (rc) return rc;
+            d.base = (const int*)p->add(ctx, base.data(), base.size() * 4, &rc);
+        } else {
+            d.col = p->add(ctx, scol.data(), std::max<size_t>(16, slots * 4), &rc);
+        }
+        if (rc) return rc;
+        d.data = p->add(ctx, enc.data(), std::max<size_t>(16, enc.size()), &rc);
+        if (rc) return rc;
+        if (!scale.empty()) {
+            d.scale = (const float*)p->add(ctx, scale.data(), scale.size() * 4, &rc);
+            if (rc) return rc;
+        }
+        d.stride = (int)sstride;
+        d.brows = brows;
+        d.nsl = nsl;
+        d.bh = (int)bh;
+        d.bw = (int)bw;
+        d.rows = rows;
+        d.cols = cols;
+        d.codec = codec;
+        d.delta = delta ? 1 : 0;
+        const std::vector<int2> win = bsr_windows(sp, scol, brows, (int)bw, cols);
+        if (!win.empty()) {
+            d.win = (const int2*)p->add(ctx, win.data(), win.size() * sizeof(int2), &rc);
+            if (rc) return rc;
+        }
+        p->bsrm = d;
+        p->mixed = true;
+        char name[48];
+        if (codec == 0) {
+            std::snprintf(name, sizeof(name), "bsr%u%u_sliced%s%s", bh, bw, delta ? "_d16" : "", d.win ? "_xw" : "");
+            fill_info(p, CLSPMV_FMT_BSR, 0, rows, cols, m->matinfo.nnz, name, 1);
+        } else {
+            std::snprintf(name, sizeof(name), "bsr%u%u_mixed_%s%s%s", bh, bw, codec_name(codec), delta ? "_d16" : "",
+                          d.win ? "_xw" : "");
+            fill_info(p, CLSPMV_FMT_BSR, kMethodMixed + codec + (delta ? 8 : 0), rows, cols, m->matinfo.nnz, name, 1);
+        }
+        return CLSPMV_OK;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1195,6 +1358,9 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
         int rc = 0;
         const unsigned bs = bw * bh;
         if (method == 0 && cols < (int)bw) method = 1;  // the sliced kernel reads bw columns at a time
+        // 2x2..4x4: the sliced layout with block columns as int16 steps
+        // within a block row when every step fits (half the column bytes)
+        if (method == 0 && bw <= 4) return build_bsr_coded(ctx, p, m, 0, 1);
         if (method == 0) {
             // Sliced device layout (see bsr_sliced_kernel): built here from
             // the block-CSR host arrays.
@@ -1342,152 +1508,7 @@ int clspmv_part_bsr_mixed(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int code
                           clspmv_part_t* out) {
     return make_part(ctx, out, [&](clspmv_part_t p) {
         if (!codec_ok(codec)) return CLSPMV_E_INVALID;
-        const unsigned bw = m->bsr_bwidth, bh = m->bsr_bheight;
-        if (bw != bh || bw < 2 || bw > 4) return CLSPMV_E_UNSUPPORTED;
-        const int rows = m->matinfo.height, cols = m->matinfo.width;
-        if (cols < (int)bw) {
-            set_error("mixed BSR needs at least bw columns");
-            return CLSPMV_E_UNSUPPORTED;
-        }
-        const int brows = m->bsr_row_num, nb = m->bsr_block_num, stride = m->bsr_block_stride;
-        if (brows != (rows + (int)bh - 1) / (int)bh || stride < nb || m->bsr_row_ptr[0] != 0 ||
-            m->bsr_row_ptr[brows] != nb)
-            return CLSPMV_E_INVALID;
-        for (int r = 0; r < brows; ++r)
-            if (m->bsr_row_ptr[r + 1] < m->bsr_row_ptr[r]) return CLSPMV_E_INVALID;
-        if (!cols_ok(m->bsr_col_id, (size_t)nb, 0, std::max(1, cols))) {
-            set_error("BSR block column out of range");
-            return CLSPMV_E_INVALID;
-        }
-        const unsigned bs = bw * bh;
-        const int S = kBsrSlice, ns = (brows + S - 1) / S;
-        std::vector<int> sp((size_t)ns + 1, 0);
-        for (int s = 0; s < ns; ++s) {
-            int wmax = 0;
-            for (int r = s * S; r < std::min(brows, s * S + S); ++r)
-                wmax = std::max(wmax, m->bsr_row_ptr[r + 1] - m->bsr_row_ptr[r]);
-            if ((int64_t)sp[s] + (int64_t)wmax * S > INT32_MAX) return CLSPMV_E_TOO_LARGE;
-            sp[s + 1] = sp[s] + wmax * S;
-        }
-        const size_t slots = (size_t)sp[ns], sstride = std::max<size_t>(64, slots);
-        std::vector<int> scol(slots, 0), base((size_t)std::max(1, brows), 0);
-        std::vector<float> sdat(sstride * bs, 0.0f);
-        const std::vector<int> fill = bsr_slice_fill(m, brows);
-        for (int r = 0; r < brows; ++r) {
-            const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
-            const int a = m->bsr_row_ptr[r], e = m->bsr_row_ptr[r + 1];
-            int last = a < e ? m->bsr_col_id[a] : fill[(size_t)s];
-            base[(size_t)r] = last;
-            for (int k = 0; k < width; ++k) {
-                const size_t idx = (size_t)sp[s] + (size_t)k * S + t;
-                if (a + k < e) {
-                    last = m->bsr_col_id[a + k];
-                    for (unsigned q = 0; q < bs; ++q) sdat[(size_t)q * sstride + idx] = m->bsr_data[(size_t)q * stride + a + k];
-                }
-                scol[idx] = last;
-            }
-        }
-        // int16 deltas when every step fits, else int32 columns
-        bool delta = delta_cols != 0;
-        std::vector<int16_t> dcol;
-        if (delta) {
-            dcol.assign(slots, 0);
-            for (int r = 0; r < brows && delta; ++r) {
-                const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
-                int prev = base[(size_t)r];
-                for (int k = 0; k < width; ++k) {
-                    const size_t idx = (size_t)sp[s] + (size_t)k * S + t;
-                    const int dlt = scol[idx] - prev;
-                    if (dlt < INT16_MIN || dlt > INT16_MAX) {
-                        delta = false;
-                        break;
-                    }
-                    dcol[idx] = (int16_t)dlt;
-                    prev = scol[idx];
-                }
-            }
-        }
-        const int nsl = ns;
-        std::vector<float> scale;
-        std::vector<unsigned char> enc;
-        const size_t nv = sstride * bs;
-        if (codec == kCodecBf16x2) {
-            std::vector<uint32_t> w(nv);
-            for (size_t i = 0; i < nv; ++i) w[i] = bf16x2_word(sdat[i]);
-            enc.assign(reinterpret_cast<unsigned char*>(w.data()), reinterpret_cast<unsigned char*>(w.data()) + nv * 4);
-        } else if (codec_row_scaled(codec)) {
-            scale.assign((size_t)std::max(1, brows), 1.0f);
-            for (int r = 0; r < brows; ++r) {
-                const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
-                float amax = 0.f;
-                for (int k = 0; k < width; ++k)
-                    for (unsigned q = 0; q < bs; ++q)
-                        amax = std::max(amax, std::fabs(sdat[(size_t)q * sstride + (size_t)sp[s] + (size_t)k * S + t]));
-                scale[(size_t)r] = row_scale(codec, amax);
-            }
-            if (int e = check_row_scales(codec, scale)) return e;
-            std::vector<uint16_t> h(nv, 0);
-            for (int r = 0; r < brows; ++r) {
-                const int s = r / S, t = r % S, width = (sp[s + 1] - sp[s]) / S;
-                for (int k = 0; k < width; ++k)
-                    for (unsigned q = 0; q < bs; ++q) {
-                        const size_t i = (size_t)q * sstride + (size_t)sp[s] + (size_t)k * S + t;
-                        h[i] = row_encode(codec, sdat[i] / scale[(size_t)r]);
-                    }
-            }
-            enc.assign(reinterpret_cast<unsigned char*>(h.data()), reinterpret_cast<unsigned char*>(h.data()) + nv * 2);
-        } else {
-            scale.assign((size_t)bs * nsl, 1.0f);
-            enc.assign(nv, 0);
-            for (unsigned q = 0; q < bs; ++q)
-                for (int s = 0; s < ns; ++s) {
-                    float amax = 0.f;
-                    for (int i = sp[s]; i < sp[s + 1]; ++i) amax = std::max(amax, std::fabs(sdat[(size_t)q * sstride + i]));
-                    const float sc = fp8_scale(amax);
-                    scale[(size_t)q * nsl + s] = sc;
-                    for (int i = sp[s]; i < sp[s + 1]; ++i)
-                        enc[(size_t)q * sstride + i] = f32_to_e4m3(sdat[(size_t)q * sstride + i] / sc);
-                }
-        }
-        int rc = 0;
-        DevBsrMixed d{};
-        d.sp = (const int*)p->add(ctx, sp.data(), sp.size() * 4, &rc);
-        if (rc) return rc;
-        if (delta) {
-            d.col = p->add(ctx, dcol.data(), std::max<size_t>(16, slots * 2), &rc);
-            if (rc) return rc;
-            d.base = (const int*)p->add(ctx, base.data(), base.size() * 4, &rc);
-        } else {
-            d.col = p->add(ctx, scol.data(), std::max<size_t>(16, slots * 4), &rc);
-        }
-        if (rc) return rc;
-        d.data = p->add(ctx, enc.data(), std::max<size_t>(16, enc.size()), &rc);
-        if (rc) return rc;
-        if (!scale.empty()) {
-            d.scale = (const float*)p->add(ctx, scale.data(), scale.size() * 4, &rc);
-            if (rc) return rc;
-        }
-        d.stride = (int)sstride;
-        d.brows = brows;
-        d.nsl = nsl;
-        d.bh = (int)bh;
-        d.bw = (int)bw;
-        d.rows = rows;
-        d.cols = cols;
-        d.codec = codec;
-        d.delta = delta ? 1 : 0;
-        const std::vector<int2> win = bsr_windows(sp, scol, brows, (int)bw, cols);
-        if (!win.empty()) {
-            d.win = (const int2*)p->add(ctx, win.data(), win.size() * sizeof(int2), &rc);
-            if (rc) return rc;
-        }
-        p->bsrm = d;
-        p->mixed = true;
-        char name[48];
-        std::snprintf(name, sizeof(name), "bsr%u%u_mixed_%s%s%s", bh, bw, codec_name(codec), delta ? "_d16" : "",
-                      d.win ? "_xw" : "");
-        fill_info(p, CLSPMV_FMT_BSR, kMethodMixed + codec + (delta ? 8 : 0), rows, cols, m->matinfo.nnz, name, 1);
-        return CLSPMV_OK;
+        return build_bsr_coded(ctx, p, m, codec, delta_cols);
     });
 }
 

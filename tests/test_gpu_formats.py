@@ -226,6 +226,21 @@ def test_bsr_ragged(ctx, method):
     _check(coo, y, x, y0, f"BSR ragged {info['kernel']}")
 
 
+def test_bsr_sliced_int16_steps_and_fallback(ctx):
+    # FP32 sliced BSR stores block columns as int16 steps within a block row
+    # when every step fits; a step of 100000 columns keeps int32 columns
+    near = M.gen_block_rows(3000, 3, 6, 500, 22)
+    far_r = np.array([0, 0, 0, 1, 1, 1], np.int32)
+    far_c = np.array([0, 1, 100000, 0, 1, 100001], np.int32)
+    far = M.CooMatrix.from_arrays(far_r, far_c, np.arange(1, 7, dtype=np.float32), 100004, 2)
+    for coo, tag in ((near, "_d16"), (far, "")):
+        x, y0 = _x_y(coo)
+        y, info = gpu_spmv(ctx, L.FMT_BSR, M.coo2bsr(coo, 3, 3), x, y0, 0)
+        assert info["kernel"].startswith("bsr33_sliced" + tag) and info["method"] == 0, info
+        assert ("_d16" in info["kernel"]) == bool(tag), info
+        _check(coo, y, x, y0, f"BSR int16 steps {info['kernel']}")
+
+
 @pytest.mark.parametrize("lanes", [2, 4, 8, 16])
 def test_bsr_explicit_lanes(ctx, lanes):
     # method L = lanes per block row (16 shares its number with no other kind)
@@ -267,7 +282,9 @@ def test_bsr_lds_window_matches_plain(ctx, b, name, make):
         yp, ip = gpu_spmv(ctx, L.FMT_BSR, bsr, x, y0, 0)
     finally:
         _bsr_window(1)
-    assert iw["kernel"] == f"bsr{b}{b}_sliced_xw" and ip["kernel"] == f"bsr{b}{b}_sliced", (iw, ip)
+    # names: bsr<b><b>_sliced[_d16][_xw] (int16 column steps where they fit)
+    assert iw["kernel"].startswith(f"bsr{b}{b}_sliced") and iw["kernel"].endswith("_xw"), iw
+    assert ip["kernel"].startswith(f"bsr{b}{b}_sliced") and not ip["kernel"].endswith("_xw"), ip
     assert np.array_equal(yw.view(np.uint32), yp.view(np.uint32)), name
     _check(coo, yw, x, y0, f"BSR {name} {iw['kernel']}")
 
@@ -280,7 +297,7 @@ def test_bsr_window_out_of_range_uses_plain_kernel(ctx, name, make):
     coo = make()
     x, y0 = _x_y(coo)
     y, info = gpu_spmv(ctx, L.FMT_BSR, M.coo2bsr(coo, 2, 2), x, y0, 0)
-    assert info["kernel"] == "bsr22_sliced", info
+    assert info["kernel"].startswith("bsr22_sliced") and not info["kernel"].endswith("_xw"), info
     _check(coo, y, x, y0, f"BSR {name} {info['kernel']}")
 
 
