@@ -318,9 +318,11 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
  *   COO      0 segmented reduction, 2 column-sliced, 4 LDS x-tiled
  *   DIA      0 one row per lane, 1 four rows per lane
  *   BDIA     0 direct x reads, 1 LDS-staged x band
- *   BSR      0 sliced (64 block rows per slice; with a wide band, x
- *            windows of 1024 block rows staged in LDS, kernel name
- *            suffix "_xw"), 1 (or L = 2..16) L lanes per block row
+ *   BSR      0 sliced (64 block rows per slice; for r x r blocks with
+ *            2 <= r <= 4, block columns stored as int16 steps within a
+ *            block row when every step fits, kernel name suffix "_d16";
+ *            with a wide band, x windows of 1024 block rows staged in
+ *            LDS, suffix "_xw"), 1 (or L = 2..16) L lanes per block row
  * "Column-sliced" cuts the columns into slices whose part of x (<= 2 MiB by
  * default) stays in one XCD's L2; one launch per slice adds into y.
  * "LDS x-tiled" gives each 1024-thread workgroup a block of <= 32768 rows
