@@ -43,31 +43,23 @@ def log(msg):
 
 
 def cpu_baseline(workloads, budget_s=12.0):
-    """Single-core reference CPU SpMV on the same matrices (bounded)."""
-    import ctypes as C
-
-    import numpy as np
-
+    """The reference's CPU SpMV (spmv_only -> coo_spmv, A/src/util.cpp:85-91,
+    A/include/spmv_serial.h:184-196) as restated in oracle/spmv_oracle.c
+    (bit-exact with the reference's compiled spmv_only on the golden
+    matrices), one core, on a bounded sample of the same matrices."""
     from oracle import oracle as O
 
-    R = O.ref()
-    kind = "reference" if R is not None else "port"
     total_flops = 0.0
     total_t = 0.0
     reps_done = []
     for w in workloads:
         coo = w["coo"]
         x = w["x"]
-        y = np.zeros(coo.height, np.float32)
         reps = 0
         t_w = 0.0
         while reps == 0 or (t_w < budget_s / len(workloads) and reps < 100):
             t0 = time.perf_counter()
-            if R is not None:
-                R.ref_spmv_only(C.addressof(coo.s), x.ctypes.data_as(C.POINTER(C.c_float)),
-                                y.ctypes.data_as(C.POINTER(C.c_float)))
-            else:
-                y = O.spmv_only(coo.row_id, coo.col_id, coo.data, x, coo.height)
+            O.spmv_only(coo.row_id, coo.col_id, coo.data, x, coo.height)
             t_w += time.perf_counter() - t0
             reps += 1
         total_t += t_w
@@ -83,8 +75,9 @@ def cpu_baseline(workloads, budget_s=12.0):
         "value": round(total_flops / total_t / 1e9, 4),
         "unit": "GFLOP/s",
         "cores": 1,
-        "kind": kind,
-        "sample": "spmv_only (y=0; y[row]+=val*x[col] in COO order, FP32) over the full matrices: "
+        "kind": "port",
+        "sample": "oracle_spmv_only, the restated reference spmv_only (y=0; y[row]+=val*x[col] in COO order, "
+                  "FP32, no FMA), one thread, over the full matrices: "
                   + ", ".join(reps_done) + f"; {total_t:.1f} s; host {os.cpu_count()} CPUs, {model}",
     }
     # project CPU baseline (not the reference): OpenMP CSR on all host cores
@@ -220,12 +213,19 @@ def mixed_c3(ctx, args, work, per):
         dy.free()
         return y
 
-    def time_part(part, dx, dy):
-        part.spmv(dx, dy, 2)
+    def time_part(part, dx, dy, copies=()):
+        """HIP-event time per launch, rotating over (part, x, y) copies when
+        given (a working set above the Infinity Cache)."""
+        rot = [(part, dx, dy)] + list(copies)
+        for p_, x_, y_ in rot:
+            p_.spmv(x_, y_, 1)
         ctx.sync()
+        kr = max(1, k // len(rot)) * len(rot)
         ctx.time_begin()
-        part.spmv(dx, dy, k)
-        return ctx.time_end() / k
+        for i in range(kr):
+            p_, x_, y_ = rot[i % len(rot)]
+            p_.spmv(x_, y_, 1)
+        return ctx.time_end() / kr
 
     for w in work:
         coo, name = w["coo"], w["name"]
@@ -277,10 +277,13 @@ def mixed_c3(ctx, args, work, per):
         e_max, e_norm = _errors(gpu.result_to_cpu(), y64, srow)
         infos = gpu.part_infos()
         ms32 = ms if codec == 0 else ms32
+        # the Cocktail driver's error guard (default 1e-4) keeps a run's
+        # codec only when its codec_bound allows; the others run in FP32
         rec[label] = {"kernel": "Cocktail: " + "+".join(p["kernel"] for p in infos), "ms": round(ms, 5),
                       "stored_bytes": int(sum(p["stored_bytes"] for p in infos)),
                       "eff_gnnz_per_s": round(coo.nnz / (ms * 1e-3) / 1e9, 1),
-                      "max_rel_err": e_max, "norm_rel_err": e_norm}
+                      "max_rel_err": e_max, "norm_rel_err": e_norm,
+                      "codec_bound": max([p["codec_bound"] for p in infos] + [0.0])}
         if codec:
             rec[label]["speedup_vs_fp32"] = round(ms32 / ms, 3)
             log(f"c3 c2 {rec[label]['kernel']}: {ms * 1e3:.1f} us ({ms32 / ms:.2f}x FP32 Cocktail), "
@@ -300,14 +303,26 @@ def _codec_labels(L):
 
 def _mixed_codecs(ctx, M, L, rec, fmt, mat, coo, w, ms32, y64, srow, time_part, run_once, name=None):
     """Adds one record per value codec to rec (time, bytes, speedup, errors)."""
+    import numpy as np
+
     for codec, label in _codec_labels(L):
         part = M.MixedPart(ctx, fmt, mat, codec, True)
-        ms = time_part(part, w["dx"], w["dy"])
+        # copies so that the timed launches rotate over > 600 MB
+        work_set = int(part.info["stored_bytes"]) + 4 * coo.width + 8 * coo.height
+        copies = [(M.MixedPart(ctx, fmt, mat, codec, True), M.DeviceArray.from_numpy(ctx, w["x"]),
+                   M.DeviceArray.from_numpy(ctx, np.zeros(coo.height, np.float32)))
+                  for _ in range(max(0, min(4, -(-600_000_000 // work_set) - 1)))]
+        ms = time_part(part, w["dx"], w["dy"], copies)
+        for p_, x_, y_ in copies:
+            p_.free()
+            x_.free()
+            y_.free()
         e_max, e_norm = _errors(run_once(part, w["dx"], coo.height), y64, srow)
         rec[label] = {"kernel": part.info["kernel"], "ms": round(ms, 5),
                       "stored_bytes": int(part.info["stored_bytes"]),
                       "eff_gnnz_per_s": round(coo.nnz / (ms * 1e-3) / 1e9, 1),
-                      "speedup_vs_fp32": round(ms32 / ms, 3), "max_rel_err": e_max, "norm_rel_err": e_norm}
+                      "speedup_vs_fp32": round(ms32 / ms, 3), "max_rel_err": e_max, "norm_rel_err": e_norm,
+                      "codec_bound": part.info["codec_bound"]}
         log(f"c3 {name or w.get('name')} {part.info['kernel']}: {ms * 1e3:.1f} us ({ms32 / ms:.2f}x FP32), "
             f"{part.info['stored_bytes'] / 1e6:.1f} MB, max rel err {e_max:.2e}, norm rel err {e_norm:.2e}")
         part.free()
@@ -315,12 +330,18 @@ def _mixed_codecs(ctx, M, L, rec, fmt, mat, coo, w, ms32, y64, srow, time_part, 
 
 
 def _best_within(rec, tol):
-    """rec["best_max_err_1e-4"]: the fastest codec whose componentwise error
-    is at most tol, with its speedup over FP32."""
-    ok = [(r["ms"], lab) for lab, r in rec.items() if lab != "fp32" and r["max_rel_err"] <= tol]
-    if ok:
-        ms, lab = min(ok)
-        rec["best_max_err_1e-4"] = {"codec": lab, "speedup_vs_fp32": rec[lab]["speedup_vs_fp32"]}
+    """rec["best_max_err_1e-4"]: the fastest codec whose measured
+    componentwise error (this x) is at most tol; rec["best_guaranteed_1e-4"]:
+    the fastest whose codec_bound -- the encoding's bound on the
+    componentwise error for any x -- is at most tol (each with its speedup
+    over FP32; absent when no codec qualifies)."""
+    for key, ok in (("best_max_err_1e-4", lambda r: r["max_rel_err"] <= tol),
+                    ("best_guaranteed_1e-4", lambda r: r.get("codec_bound", 1.0) <= tol and r["max_rel_err"] <= tol)):
+        cand = [(r["ms"], lab) for lab, r in rec.items() if lab != "fp32" and isinstance(r, dict) and "ms" in r
+                and ok(r)]
+        if cand:
+            ms, lab = min(cand)
+            rec[key] = {"codec": lab, "speedup_vs_fp32": rec[lab]["speedup_vs_fp32"]}
 
 
 def batched_c4(ctx, args):
@@ -503,14 +524,24 @@ def batched_c4(ctx, args):
     return out
 
 
-def load_traffic(kernel):
+def load_traffic(kernel, layout_bytes):
+    """HBM bytes per launch of `kernel` from the rocprofv3 counter runs
+    summarised in profiles/ (FETCH_SIZE and WRITE_SIZE, each in a run of its
+    own, scaled by what the counters read for a plain copy kernel).  The
+    counters cannot be read inside this process, so the figure is the one
+    recorded for the same kernel and layout; it is dropped (None) when the
+    layout's bytes differ from those recorded with it (a changed layout)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         t = json.load(f)
     v = t.get(kernel)
-    return v.get("hbm_bytes_per_launch") if isinstance(v, dict) else None
+    if not isinstance(v, dict) or v.get("alg_bytes") != layout_bytes:
+        return None, None
+    src = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of clspmv_amd.kprof on the same matrix and layout "
+           f"({v.get('summary')}, measured {t.get('_calibration', {}).get('measured', 'with the summary')})")
+    return v.get("hbm_bytes_per_launch"), src
 
 
 def main():
@@ -546,9 +577,19 @@ def main():
         x = M.gen_vector(wl.coo.width, 7)
         dx = M.DeviceArray.from_numpy(ctx, x)
         dy = M.DeviceArray.from_numpy(ctx, np.zeros(wl.coo.height, np.float32))
+        # copies of the part, x and y so that the per-kernel timing below can
+        # rotate over a working set above 2 x the 256 MB Infinity Cache
+        extra = []
+        for _ in range(max(0, min(3, -(-600_000_000 // int(part.info["alg_bytes"])) - 1))):
+            extra.append((M.Part(ctx, wl.fmt, wl.mat, wl.method), M.DeviceArray.from_numpy(ctx, x),
+                          M.DeviceArray.from_numpy(ctx, np.zeros(wl.coo.height, np.float32))))
         # format arrays are on the device now; keep only the COO for the CPU leg
         wl.mat.free()
-        work.append({"name": n, "coo": wl.coo, "x": x, "part": part, "dx": dx, "dy": dy, "desc": wl.describe})
+        # algorithmic bytes: the reference format's bytes for this matrix, or
+        # the bytes this layout moves when that is less
+        alg = min(W.reference_bytes(n, wl.coo), int(part.info["alg_bytes"]))
+        work.append({"name": n, "coo": wl.coo, "x": x, "part": part, "dx": dx, "dy": dy, "desc": wl.describe,
+                     "alg": alg, "ref_bytes": W.reference_bytes(n, wl.coo), "copies": extra})
         log(f"{n}: {wl.describe} rows={wl.coo.height} nnz={wl.coo.nnz} kernel={part.info['kernel']} "
             f"alg_bytes={part.info['alg_bytes']} ({time.perf_counter() - t0:.1f}s setup)")
 
@@ -566,7 +607,7 @@ def main():
     ctx.sync()
     ms_step = ms_total / args.steps
     flops = sum(2.0 * w["coo"].nnz for w in work)
-    bytes_alg = sum(w["part"].info["alg_bytes"] for w in work)
+    bytes_alg = sum(w["alg"] for w in work)
     gflops = flops / (ms_step * 1e-3) / 1e9
     gbps = bytes_alg / (ms_step * 1e-3) / 1e9
     log(f"step: {ms_step:.4f} ms  {gflops:.1f} GFLOP/s  {gbps:.1f} GB/s")
@@ -579,13 +620,29 @@ def main():
         ctx.sync()
         ctx.time_begin()
         w["part"].spmv(w["dx"], w["dy"], k)
-        ms = ctx.time_end() / k
+        ms_one = ctx.time_end() / k
+        # the same launches rotating over the copies (working set > 512 MB),
+        # so that no launch finds its matrix in the Infinity Cache
+        rot = [(w["part"], w["dx"], w["dy"])] + w["copies"]
+        ms = ms_one
+        if len(rot) > 1:
+            kr = k // len(rot) * len(rot)
+            ctx.sync()
+            ctx.time_begin()
+            for i in range(kr):
+                p_, x_, y_ = rot[i % len(rot)]
+                p_.spmv(x_, y_, 1)
+            ms = ctx.time_end() / kr
         info = w["part"].info
+        alg = w["alg"]
         per[w["name"]] = {
             "format": w["desc"], "kernel": info["kernel"], "rows": info["rows"], "nnz": info["nnz"],
-            "ms": round(ms, 5), "gflops": round(2.0 * info["nnz"] / (ms * 1e-3) / 1e9, 1),
-            "alg_bytes": info["alg_bytes"], "gbps": round(info["alg_bytes"] / (ms * 1e-3) / 1e9, 1),
-            "pct_hbm_peak": round(100.0 * info["alg_bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1),
+            "ms": round(ms, 5), "ms_single_copy": round(ms_one, 5), "rotated_copies": len(rot),
+            "rotated_working_set_bytes": int(len(rot) * info["alg_bytes"]),
+            "gflops": round(2.0 * info["nnz"] / (ms * 1e-3) / 1e9, 1),
+            "alg_bytes": alg, "reference_format_bytes": w["ref_bytes"], "layout_bytes": info["alg_bytes"],
+            "gbps": round(alg / (ms * 1e-3) / 1e9, 1),
+            "pct_hbm_peak": round(100.0 * alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1),
             # CSR-equivalent bytes (8 per nonzero + row pointers + x + y) and
             # stored bytes per nonzero of the format (its padding included)
             "csr_equiv_bytes": int(8 * info["nnz"] + 4 * (info["rows"] + 1) + 4 * info["cols"] + 8 * info["rows"]),
@@ -611,11 +668,12 @@ def main():
         per[n]["pct_of_copy"] = round(100.0 * per[n]["gbps"] / copy_gbps, 1)
 
     dom = max(per, key=lambda n: per[n]["ms"])
+    traffic, traffic_src = load_traffic(per[dom]["kernel"], per[dom]["layout_bytes"])
     d = per[dom]
     achieved = d["alg_bytes"] / (d["ms"] * 1e-3) / 1e9
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": load_traffic(d["kernel"]),
+        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "traffic_source": traffic_src,
         "kernel": d["kernel"], "matrix": dom, "alg_bytes_per_launch": d["alg_bytes"],
     }
 
@@ -634,6 +692,10 @@ def main():
         w["part"].free()
         w["dx"].free()
         w["dy"].free()
+        for p_, x_, y_ in w["copies"]:
+            p_.free()
+            x_.free()
+            y_.free()
     ctx.close()
 
     out = {

@@ -113,7 +113,7 @@ class MaxflopInfo(C.Structure):
 class PartInfo(C.Structure):
     _fields_ = [("format", C.c_int), ("method", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
                 ("nnz", C.c_longlong), ("stored_bytes", C.c_longlong), ("alg_bytes", C.c_longlong),
-                ("launches", C.c_int), ("kernel", C.c_char * 64)]
+                ("launches", C.c_int), ("kernel", C.c_char * 64), ("codec_bound", C.c_double)]
 
 
 # (name, restype, argtypes)
@@ -193,6 +193,9 @@ _SIGS = [
     ("clspmv_init_mat_kernels", C.c_int, [C.c_void_p, _P(Cocktail), _FP, _FP, C.c_int, _P(C.c_void_p)]),
     ("clspmv_free_mat_kernels", C.c_int, [C.c_void_p]),
     ("clspmv_do_spmv", C.c_int, [C.c_void_p, C.c_int]),
+    ("clspmv_power_iterate", C.c_int, [C.c_void_p, C.c_int]),
+    ("clspmv_evaluate", C.c_int, [C.c_void_p, _FP, _FP, _FP]),
+    ("clspmv_evaluate_report", C.c_int, [C.c_void_p, C.c_void_p, _FP, _FP, _FP, C.c_int, C.c_int, C.c_void_p]),
     ("clspmv_cpy_vector_from_cpu", C.c_int, [C.c_void_p, _FP, C.c_int]),
     ("clspmv_cpy_vector_from_gpu", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     ("clspmv_cpy_result_from_cpu", C.c_int, [C.c_void_p, _FP, C.c_int]),
@@ -229,6 +232,7 @@ _SIGS = [
     ("clspmv_probe_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, _FP]),
     ("clspmv_probe_launch", C.c_int, [C.c_void_p, C.c_int, _FP]),
     ("clspmv_probe_launch_grid", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _FP]),
+    ("clspmv_probe_graph_launch", C.c_int, [C.c_void_p, C.c_int, C.c_int, _FP]),
 ]
 
 _lib = None

@@ -165,10 +165,37 @@ class GpuCocktail:
 
 
 def power_iterate(ctx: Context, gpu: GpuCocktail, k: int):
-    """y = A^k x on the device (A/src/eval.cpp:19-72 `example`): after each
-    do_spmv the result is copied into the vector buffer on the device."""
+    """The iteration of A/src/eval.cpp:19-72 (`example`): k times
+    {y += A x; x := y}, device-resident (clspmv_power_iterate: captured
+    graphs, no host round trip); returns y."""
+    check(lib().clspmv_power_iterate(gpu.h, k), "power_iterate")
+    return gpu.result_to_cpu()
+
+
+def power_iterate_host(ctx: Context, gpu: GpuCocktail, k: int):
+    """The same iteration driven from the host as the reference's example
+    does it: one do_spmv, then cpy_vector_from_gpu(y), k times."""
     _, dy = gpu.buffers()
     for _ in range(k):
         gpu.do_spmv(1)
         gpu.vector_from_gpu(dy, gpu.width)
     return gpu.result_to_cpu()
+
+
+class EvalReport(C.Structure):
+    _fields_ = [("distance", C.c_double), ("max_diff", C.c_double), ("max_diff_ratio", C.c_double),
+                ("max_rel_err", C.c_double), ("max_diff_id", C.c_int), ("ntimes", C.c_int), ("nparts", C.c_int),
+                ("ms_per_spmv", C.c_double), ("gflops", C.c_double), ("alg_gbps", C.c_double)]
+
+
+def evaluate(ctx: Context, mat: CocktailMatrix, vec, res, coores, ntimes: int = 1000, verbose: bool = False):
+    """evaluate (A/src/spmv_cocktail.cpp:1431-1455) on a context: upload,
+    one checked do_spmv against coores, ntimes timed; returns the report."""
+    fp = C.POINTER(C.c_float)
+    v = np.ascontiguousarray(vec, np.float32)
+    r = np.ascontiguousarray(res, np.float32)
+    c = np.ascontiguousarray(coores, np.float32)
+    rep = EvalReport()
+    check(lib().clspmv_evaluate_report(ctx.h, C.byref(mat.s), v.ctypes.data_as(fp), r.ctypes.data_as(fp),
+                                       c.ctypes.data_as(fp), ntimes, int(verbose), C.byref(rep)), "evaluate")
+    return {k: getattr(rep, k) for k, _ in EvalReport._fields_}

@@ -12,7 +12,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
 #include <vector>
 
@@ -29,7 +31,11 @@ struct clspmv_gpumat_s {
     int width = 0, height = 0;
     std::vector<clspmv_part_t> parts;  // in do_spmv order
     std::vector<int> row0;             // first row of each part (row-range parts of the flat rows)
+    // captured graphs: key = passes per graph * 2 + (1: each pass followed
+    // by x := y, the iteration of A/src/eval.cpp:19-72)
+    std::map<int, hipGraphExec_t> graphs;
     ~clspmv_gpumat_s() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         for (auto p : parts) clspmv_part_free(p);
         x.release();
         y.release();
@@ -39,6 +45,88 @@ struct clspmv_gpumat_s {
 namespace {
 
 constexpr int kXtile = 4;  // method number of the x-tiled layout
+
+// do_spmv and power_iterate replay captured hipGraphs of up to kGraphPasses
+// passes (one pass = every part's launch in order) instead of launching
+// each part from the host; "cocktail_graph" 0 launches directly.
+bool g_use_graph = true;
+constexpr int kGraphPasses = 64;
+
+// x[i] = y[i] for i < n: the vector update of one iteration.
+__global__ void copy_vec_kernel(const float* __restrict__ src, float* __restrict__ dst, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// Enqueues `passes` passes on stream st, each followed by x := y when
+// `iterate`.
+int enqueue_passes(clspmv_gpumat_s* g, int passes, bool iterate, hipStream_t st) {
+    const float* x = (const float*)g->x.ptr;
+    float* y = (float*)g->y.ptr;
+    const int n = std::min(g->width, g->height);
+    for (int i = 0; i < passes; ++i) {
+        for (size_t j = 0; j < g->parts.size(); ++j) {
+            const int e = clspmv_part_launch(g->parts[j], x, y + g->row0[j], st);
+            if (e) return hip_status((hipError_t)e, "do_spmv launch");
+        }
+        if (iterate && n > 0) {
+            hipLaunchKernelGGL(copy_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, st, y, (float*)g->x.ptr, n);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return hip_status(e, "copy launch");
+        }
+    }
+    return CLSPMV_OK;
+}
+
+// The executable graph of `passes` passes (captured on first use, on a
+// stream of its own, so that the caller's stream is never in capture mode).
+int pass_graph(clspmv_gpumat_s* g, int passes, bool iterate, hipGraphExec_t* out) {
+    const int key = passes * 2 + (iterate ? 1 : 0);
+    auto it = g->graphs.find(key);
+    if (it != g->graphs.end()) {
+        *out = it->second;
+        return CLSPMV_OK;
+    }
+    hipStream_t cs = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_status(e, "hipStreamCreate");
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int rc = e == hipSuccess ? CLSPMV_OK : hip_status(e, "hipStreamBeginCapture");
+    hipGraph_t graph = nullptr;
+    if (!rc) {
+        rc = enqueue_passes(g, passes, iterate, cs);
+        e = hipStreamEndCapture(cs, &graph);
+        if (!rc && e != hipSuccess) rc = hip_status(e, "hipStreamEndCapture");
+    }
+    hipGraphExec_t exec = nullptr;
+    if (!rc) {
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_status(e, "hipGraphInstantiate");
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cs);
+    if (rc) return rc;
+    g->graphs[key] = exec;
+    *out = exec;
+    return CLSPMV_OK;
+}
+
+// ntimes passes on the context stream: whole graphs of kGraphPasses passes,
+// then one graph of the remainder.
+int run_passes(clspmv_gpumat_s* g, int ntimes, bool iterate) {
+    hipStream_t st = g->ctx->stream();
+    if (!g_use_graph || g->parts.empty()) return enqueue_passes(g, ntimes, iterate, st);
+    for (int done = 0; done < ntimes;) {
+        const int n = std::min(kGraphPasses, ntimes - done);
+        hipGraphExec_t exec = nullptr;
+        int rc = pass_graph(g, n, iterate, &exec);
+        if (rc) return rc;
+        const hipError_t e = hipGraphLaunch(exec, st);
+        if (e != hipSuccess) return hip_status(e, "hipGraphLaunch");
+        done += n;
+    }
+    return CLSPMV_OK;
+}
 
 // Layout choice for the row-list parts (ELL, SELL, CSR, COO) when x is wider
 // than one XCD's L2 (4 MiB): the x-tiled layout (method 4) when its cost is
@@ -140,6 +228,21 @@ bool g_split_local = true;
 // x-tiled and sliced BSR ones): 0 = FP32 (default) or a CLSPMV_VAL_* code
 // ("cocktail_value_codec").  Other parts stay FP32.
 int g_run_codec = 0;
+// Error guard of the value codec: a run part keeps its codec only when the
+// encoding's bound on the componentwise error (clspmv_part_info.codec_bound,
+// valid for any x) is at most this; otherwise it is built in FP32.  Set with
+// "cocktail_codec_tol" in units of 1e-9 (default 1e-4; 0 = no guard).
+double g_codec_tol = 1e-4;
+
+// Keeps mixed part p when its codec bound passes the guard; else frees it
+// and reports false (the caller builds the FP32 part instead).
+bool codec_passes(clspmv_part_t p) {
+    clspmv_part_info inf;
+    if (g_codec_tol <= 0 || (clspmv_part_get_info(p, &inf) == CLSPMV_OK && inf.codec_bound <= g_codec_tol))
+        return true;
+    clspmv_part_free(p);
+    return false;
+}
 constexpr int kRunBlock = 32768;
 constexpr long long kLocalWindow = 65536;
 
@@ -223,6 +326,7 @@ int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char c
         clspmv_part_t p = nullptr;
         const int rc = clspmv_part_csr_mixed(g->ctx, &sub, g_run_codec, &p);
         if (rc) return rc;
+        if (!codec_passes(p)) return add_flat_part(g, sub, meth, row0);
         g->parts.push_back(p);
         g->row0.push_back(row0);
         return CLSPMV_OK;
@@ -248,8 +352,11 @@ int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char c
         const unsigned b = cls == kLocalBsr4 ? 4 : 2;
         clspmv_bsr_matrix bsr{};
         rc = clspmv_coo2bsr(&coo, &bsr, b, b);
-        if (!rc) rc = g_run_codec ? clspmv_part_bsr_mixed(g->ctx, &bsr, g_run_codec, 1, &p)
-                                  : clspmv_part_bsr(g->ctx, &bsr, 0, &p);
+        if (!rc && g_run_codec) {
+            rc = clspmv_part_bsr_mixed(g->ctx, &bsr, g_run_codec, 1, &p);
+            if (!rc && !codec_passes(p)) p = nullptr;
+        }
+        if (!rc && !p) rc = clspmv_part_bsr(g->ctx, &bsr, 0, &p);
         clspmv_free_bsr(&bsr);
     }
     clspmv_free_coo(&coo);
@@ -442,15 +549,16 @@ int clspmv_free_mat_kernels(clspmv_gpumat_t g) {
 
 int clspmv_do_spmv(clspmv_gpumat_t g, int ntimes) {
     if (!g || ntimes < 0) return CLSPMV_E_INVALID;
-    hipStream_t st = g->ctx->stream();
-    const float* x = (const float*)g->x.ptr;
-    float* y = (float*)g->y.ptr;
-    for (int i = 0; i < ntimes; ++i)
-        for (size_t j = 0; j < g->parts.size(); ++j) {
-            int e = clspmv_part_launch(g->parts[j], x, y + g->row0[j], st);
-            if (e) return hip_status((hipError_t)e, "do_spmv launch");
-        }
-    return CLSPMV_OK;
+    return run_passes(g, ntimes, false);
+}
+
+int clspmv_power_iterate(clspmv_gpumat_t g, int iterations) {
+    if (!g || iterations < 0) return CLSPMV_E_INVALID;
+    if (g->width != g->height) {
+        set_error("power_iterate needs a square matrix (%d x %d)", g->height, g->width);
+        return CLSPMV_E_INVALID;
+    }
+    return run_passes(g, iterations, true);
 }
 
 int clspmv_cpy_vector_from_cpu(clspmv_gpumat_t g, const float* vec, int n) {
@@ -487,9 +595,92 @@ int clspmv_gpumat_buffers(clspmv_gpumat_t g, void** d_vec, void** d_res) {
 
 int clspmv_gpumat_num_parts(clspmv_gpumat_t g) { return g ? (int)g->parts.size() : 0; }
 
+// evaluate (A/src/spmv_cocktail.cpp:1431-1455 with check_and_time
+// :1134-1154): upload the partition with x = vec and y = res, one checked
+// do_spmv against coores (the caller's CPU y = res + A vec), then ntimes
+// timed do_spmv.  The check reports the reference's two_vec_compare fields
+// (A/src/util.cpp:93-131: Euclidean distance, and the largest difference of
+// magnitudes and its ratio to the larger magnitude, sign-blind) and a signed
+// one, max |a - b| / max(|a|, |b|).
+int clspmv_evaluate_report(clspmv_ctx_t ctx, const clspmv_cocktail* mat, const float* vec, const float* res,
+                           const float* coores, int ntimes, int verbose, clspmv_eval_report* out) {
+    if (!ctx || !mat || !vec || !res || !coores || ntimes < 1 || !out) return CLSPMV_E_INVALID;
+    *out = clspmv_eval_report{};
+    clspmv_gpumat_t g = nullptr;
+    int rc = clspmv_init_mat_kernels(ctx, mat, vec, res, 0, &g);
+    if (rc) return rc;
+    const int h = mat->mat_height;
+    std::vector<float> y((size_t)std::max(1, h));
+    rc = clspmv_cpy_result_from_cpu(g, res, h);
+    if (!rc) rc = clspmv_do_spmv(g, 1);
+    if (!rc) rc = clspmv_cpy_result_to_cpu(g, y.data(), h);
+    if (!rc) {
+        double sum = 0.0;
+        for (int i = 0; i < h; ++i) {
+            const double a = coores[i], b = y[(size_t)i];
+            sum += (a - b) * (a - b);
+            const double d = std::fabs(std::fabs(a) - std::fabs(b)), mx = std::max(std::fabs(a), std::fabs(b));
+            if (d > out->max_diff) {
+                out->max_diff = d;
+                out->max_diff_id = i;
+            }
+            if (mx > 0) {
+                out->max_diff_ratio = std::max(out->max_diff_ratio, d / mx);
+                out->max_rel_err = std::max(out->max_rel_err, std::fabs(a - b) / mx);
+            }
+        }
+        out->distance = std::sqrt(sum);
+    }
+    float ms = 0.f;
+    if (!rc) rc = clspmv_synchronize(ctx);
+    if (!rc) rc = clspmv_event_elapsed_begin(ctx);
+    if (!rc) rc = clspmv_do_spmv(g, ntimes);
+    if (!rc) rc = clspmv_event_elapsed_end(ctx, &ms);
+    if (!rc) {
+        long long alg = 4LL * mat->mat_width + 8LL * h;
+        for (size_t i = 0; i < g->parts.size(); ++i) {
+            clspmv_part_info inf;
+            if (clspmv_part_get_info(g->parts[i], &inf) == CLSPMV_OK) alg += inf.stored_bytes;
+        }
+        out->ntimes = ntimes;
+        out->nparts = (int)g->parts.size();
+        out->ms_per_spmv = (double)ms / ntimes;
+        out->gflops = 2.0 * mat->mat_nnz / (out->ms_per_spmv * 1e-3) / 1e9;
+        out->alg_gbps = (double)alg / (out->ms_per_spmv * 1e-3) / 1e9;
+        if (verbose) {
+            std::printf("Max diff id %d coo res %f res %f \n", out->max_diff_id, h ? coores[out->max_diff_id] : 0.f,
+                        h ? y[(size_t)out->max_diff_id] : 0.f);
+            std::printf("\nCorrectness Check: Distance %e max diff %e max diff ratio %e vec size %d\n", out->distance,
+                        out->max_diff, out->max_diff_ratio, h);
+            std::printf("\nCocktail gpu time %lf ms GFLOPS %lf\n\n", out->ms_per_spmv, out->gflops);
+        }
+    }
+    const int rf = clspmv_free_mat_kernels(g);
+    return rc ? rc : rf;
+}
+
+int clspmv_evaluate(const clspmv_cocktail* mat, const float* vec, const float* res, const float* coores) {
+    clspmv_ctx_t ctx = nullptr;
+    int rc = clspmv_init(0, &ctx);
+    if (rc) return rc;
+    clspmv_eval_report rep;
+    rc = clspmv_evaluate_report(ctx, mat, vec, res, coores, 1000, 1, &rep);
+    const int rf = clspmv_finalize(ctx);
+    return rc ? rc : rf;
+}
+
 __attribute__((visibility("hidden"))) int clspmv_cocktail_option(const char* name, long long value) {
     if (name && std::strcmp(name, "cocktail_merge_flat") == 0) {
         g_merge_flat = value != 0;
+        return CLSPMV_OK;
+    }
+    if (name && std::strcmp(name, "cocktail_codec_tol") == 0) {
+        if (value < 0) return CLSPMV_E_INVALID;
+        g_codec_tol = (double)value * 1e-9;
+        return CLSPMV_OK;
+    }
+    if (name && std::strcmp(name, "cocktail_graph") == 0) {
+        g_use_graph = value != 0;
         return CLSPMV_OK;
     }
     if (name && std::strcmp(name, "cocktail_split_local") == 0) {

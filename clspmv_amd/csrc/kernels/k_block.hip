@@ -290,6 +290,17 @@ __global__ __launch_bounds__(256) void copy_probe_kernel(const float4* __restric
 
 __global__ void empty_kernel() {}
 
+// Counts the 16-byte words where a and b differ (the copy probe's check).
+__global__ __launch_bounds__(256) void compare_probe_kernel(const uint4* __restrict__ a, const uint4* __restrict__ b,
+                                                            size_t n4, unsigned long long* __restrict__ count) {
+    unsigned long long bad = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256) {
+        const uint4 u = a[i], v = b[i];
+        bad += (u.x != v.x) | (u.y != v.y) | (u.z != v.z) | (u.w != v.w);
+    }
+    if (bad) atomicAdd(count, bad);
+}
+
 }  // namespace dev
 
 #define BH_W4_SWITCH(BH, W4, MACRO)     \
@@ -375,10 +386,9 @@ static void launch_bsr_sliced(const DevBsr& m, const float* x, float* y, hipStre
 int launch_bsr(const DevBsr& m, const float* x, float* y, hipStream_t st) {
     if (m.brows == 0) return 0;
     if (m.lanes == 0) {
+        // 2x2..4x4 sliced parts are built with the coded layout (int16
+        // column steps) and launched by launch_bsr_mixed
         switch (m.bh * 10 + m.bw) {
-            case 22: launch_bsr_sliced<2, 2>(m, x, y, st); break;
-            case 33: launch_bsr_sliced<3, 3>(m, x, y, st); break;
-            case 44: launch_bsr_sliced<4, 4>(m, x, y, st); break;
             case 55: launch_bsr_sliced<5, 5>(m, x, y, st); break;
             case 66: launch_bsr_sliced<6, 6>(m, x, y, st); break;
             default: return (int)hipErrorInvalidValue;
@@ -399,6 +409,14 @@ int launch_bsr(const DevBsr& m, const float* x, float* y, hipStream_t st) {
 int launch_copy_probe(const float4* src, float4* dst, size_t n4, hipStream_t st) {
     if (n4 == 0) return 0;
     hipLaunchKernelGGL(dev::copy_probe_kernel, dim3((unsigned)((n4 + 1023) / 1024)), dim3(256), 0, st, src, dst, n4);
+    return (int)hipGetLastError();
+}
+
+int launch_compare_probe(const void* a, const void* b, size_t n4, unsigned long long* count, hipStream_t st) {
+    if (n4 == 0) return 0;
+    const unsigned groups = (unsigned)std::min<size_t>(2048, (n4 + 255) / 256);
+    hipLaunchKernelGGL(dev::compare_probe_kernel, dim3(groups), dim3(256), 0, st, (const uint4*)a, (const uint4*)b, n4,
+                       count);
     return (int)hipGetLastError();
 }
 

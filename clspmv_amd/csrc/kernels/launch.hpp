@@ -200,6 +200,7 @@ int launch_spmm(const DevSpmm& b, hipStream_t st);
 int launch_dia_mixed(const DevDiaMixed& m, const float* x, float* y, hipStream_t st);
 int launch_bsr_mixed(const DevBsrMixed& m, const float* x, float* y, hipStream_t st);
 int launch_copy_probe(const float4* src, float4* dst, size_t n4, hipStream_t st);
+int launch_compare_probe(const void* a, const void* b, size_t n4, unsigned long long* count, hipStream_t st);
 int launch_empty(hipStream_t st, int groups = 1, int threads = 64);
 
 // Constants of the adaptive CSR and COO kernels, shared with the host

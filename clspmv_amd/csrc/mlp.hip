@@ -127,6 +127,7 @@ struct clspmv_mlp_s {
     int in = 0, hid = 0, cls = 0;
     long long np = 0;
     DevBuf p, g, m, v;          // parameters, gradients, Adam moments
+    long long t = 0;            // Adam steps taken since creation or the last set_params
     DevBuf x, y, h1, h2, z;     // activations for the current n
     DevBuf d1, d2, dz, loss_i, loss;
     int cap = 0;                // samples the activation buffers hold
@@ -259,9 +260,15 @@ int clspmv_mlp_get_params(clspmv_mlp_t m, float* params) {
     if (!m || !params) return CLSPMV_E_INVALID;
     return clspmv_memcpy_d2h(m->ctx, params, m->p.ptr, (size_t)m->np * 4);
 }
+// New parameters start a new optimisation: the Adam moments and step count
+// are reset with them.
 int clspmv_mlp_set_params(clspmv_mlp_t m, const float* params) {
     if (!m || !params) return CLSPMV_E_INVALID;
-    return clspmv_memcpy_h2d(m->ctx, m->p.ptr, params, (size_t)m->np * 4);
+    int rc = clspmv_memcpy_h2d(m->ctx, m->p.ptr, params, (size_t)m->np * 4);
+    if (!rc) rc = hip_status(hipMemsetAsync(m->m.ptr, 0, (size_t)m->np * 4, m->st()), "hipMemset");
+    if (!rc) rc = hip_status(hipMemsetAsync(m->v.ptr, 0, (size_t)m->np * 4, m->st()), "hipMemset");
+    if (!rc) m->t = 0;
+    return rc;
 }
 
 static int check_labels(const clspmv_mlp_s* m, const int* labels, int n) {
@@ -292,10 +299,14 @@ int clspmv_mlp_train(clspmv_mlp_t m, const float* x, const int* labels, int n, i
     int rc = check_labels(m, labels, n);
     if (!rc) rc = m->upload(x, labels, n);
     if (rc) return rc;
-    for (int t = 1; t <= steps; ++t) {
+    // the step count continues across calls, as the moments do, so that a
+    // run split over several calls takes the same steps as one call
+    for (int s = 0; s < steps; ++s) {
         m->forward(n);
         m->backward(n);
-        const float c1 = 1.f - std::pow(beta1, (float)t), c2 = 1.f - std::pow(beta2, (float)t);
+        const long long t = ++m->t;
+        const float c1 = (float)(1.0 - std::pow((double)beta1, (double)t));
+        const float c2 = (float)(1.0 - std::pow((double)beta2, (double)t));
         hipLaunchKernelGGL(adam_kernel, blocks(m->np), dim3(256), 0, m->st(), (float*)m->p.ptr,
                            (const float*)m->g.ptr, (float*)m->m.ptr, (float*)m->v.ptr, m->np, lr, beta1, beta2, eps,
                            c1, c2);

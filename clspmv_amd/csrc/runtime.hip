@@ -76,6 +76,7 @@ struct clspmv_part_s {
     DevCsrAdaptive csra{};
     DevCsrSliced csrs{};
     long long carry_bytes = 0;
+    double codec_bound = 0.0;  // x-tiled encoder's bound, copied into info by finish_xtile
     int csr_lanes = 0;
     DevCoo coo{};
     DevDia dia{};
@@ -188,6 +189,25 @@ int check_row_scales(int codec, const std::vector<float>& s) {
 uint16_t row_encode(int codec, float v_over_s) {
     if (codec == kCodecFp16Row) return f32_to_f16(v_over_s);
     return (uint16_t)(int16_t)std::max(-32767.0f, std::min(32767.0f, std::nearbyint(v_over_s)));
+}
+
+// Host decoders (the device's, kernels/kcodec.hpp) and the relative error
+// of one encoded value, for the codec bound of clspmv_part_info.
+float bf16x2_decode(uint32_t w) { return bf16_to_f32((uint16_t)(w >> 16)) + bf16_to_f32((uint16_t)(w & 0xffffu)); }
+float row_decode(int codec, uint16_t h) {
+    if (codec == kCodecFp16Row) {
+        _Float16 f;
+        std::memcpy(&f, &h, 2);
+        return (float)f;
+    }
+    return (float)(int16_t)h;
+}
+// |dec - v| / |v|; a value encoded as anything but itself when v == 0 (or
+// a non-finite v) counts as 1.
+double rel_err(float v, double dec) {
+    if (v == 0.0f) return dec == 0.0 ? 0.0 : 1.0;
+    if (!std::isfinite(v)) return dec == (double)v ? 0.0 : 1.0;
+    return std::min(1.0, std::fabs(dec - (double)v) / std::fabs((double)v));
 }
 
 const char* codec_name(int codec) {
@@ -305,6 +325,7 @@ int finish_xtile(clspmv_part_t p, int fmt, int rows, int cols, long long nnz) {
     std::snprintf(name, sizeof(name), "xtile%s%s_B%d_G%d", c ? "_" : "", c ? codec_name(c) : "",
                   p->xt.nblocks / p->xt.groups, p->xt.groups);
     fill_info(p, fmt, c ? kMethodMixed + c : kMethodXtile, rows, cols, nnz, name, p->xt.groups > 1 ? 2 : 1);
+    p->info.codec_bound = p->codec_bound;
     // the partial-sum buffer is scratch, not format storage
     p->info.stored_bytes -= p->carry_bytes;
     p->info.alg_bytes -= p->carry_bytes;
@@ -806,32 +827,44 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
         if (rc) return rc;
     }
     std::vector<float> vscale;
+    double bound = 0.0;  // row-scaled codecs: the scale cancels in the ratio
     if (codec == kCodecBf16x2) {
         std::vector<uint32_t> w(val.size());
-#pragma omp parallel for schedule(static) num_threads(omp_threads())
-        for (long long i = 0; i < (long long)val.size(); ++i) w[(size_t)i] = bf16x2_word(val[(size_t)i]);
+#pragma omp parallel for schedule(static) num_threads(omp_threads()) reduction(max : bound)
+        for (long long i = 0; i < (long long)val.size(); ++i) {
+            w[(size_t)i] = bf16x2_word(val[(size_t)i]);
+            bound = std::max(bound, rel_err(val[(size_t)i], bf16x2_decode(w[(size_t)i])));
+        }
         d.val = p->add(ctx, w.data(), w.size() * 4, &rc);
     } else if (codec_row_scaled(codec)) {
         std::vector<uint16_t> h(val.size());
-#pragma omp parallel for schedule(static) num_threads(omp_threads())
-        for (long long i = 0; i < (long long)val.size(); ++i) h[(size_t)i] = row_encode(codec, val[(size_t)i]);
+#pragma omp parallel for schedule(static) num_threads(omp_threads()) reduction(max : bound)
+        for (long long i = 0; i < (long long)val.size(); ++i) {
+            h[(size_t)i] = row_encode(codec, val[(size_t)i]);
+            bound = std::max(bound, rel_err(val[(size_t)i], row_decode(codec, h[(size_t)i])));
+        }
         d.val = p->add(ctx, h.data(), std::max<size_t>(h.size(), 2) * 2, &rc);
         vscale.swap(rscale);
     } else if (codec == kCodecFp8Slice) {
         const long long nc = std::max(1, tile_chunk.back());
         std::vector<uint8_t> q((size_t)nc * kXtChunk, 0);
         vscale.assign((size_t)nc, 1.0f);
-#pragma omp parallel for schedule(static) num_threads(omp_threads())
+#pragma omp parallel for schedule(static) num_threads(omp_threads()) reduction(max : bound)
         for (long long c = 0; c < nc; ++c) {
             const float* v = val.data() + (size_t)c * kXtChunk;
             float a = 0.f;
             for (int i = 0; i < kXtChunk; ++i) a = std::max(a, std::fabs(v[i]));
             const float sc = fp8_scale(a);
             vscale[(size_t)c] = sc;
-            for (int i = 0; i < kXtChunk; ++i) q[(size_t)c * kXtChunk + (size_t)i] = f32_to_e4m3(v[i] / sc);
+            for (int i = 0; i < kXtChunk; ++i) {
+                const uint8_t e = f32_to_e4m3(v[i] / sc);
+                q[(size_t)c * kXtChunk + (size_t)i] = e;
+                bound = std::max(bound, rel_err(v[i], (double)sc * e4m3_to_f32(e)));
+            }
         }
         d.val = p->add(ctx, q.data(), q.size(), &rc);
     }
+    p->codec_bound = bound;
     if (rc) return rc;
     if (!vscale.empty()) {
         d.vscale = (const float*)p->add(ctx, vscale.data(), vscale.size() * 4, &rc);
@@ -928,11 +961,15 @@ int build_bsr_coded(clspmv_ctx_t ctx, clspmv_part_t p, const clspmv_bsr_matrix* 
         std::vector<float> scale;
         std::vector<unsigned char> enc;
         const size_t nv = sstride * bs;
+        double bound = 0.0;
         if (codec == 0) {
             enc.assign(reinterpret_cast<unsigned char*>(sdat.data()), reinterpret_cast<unsigned char*>(sdat.data()) + nv * 4);
         } else if (codec == kCodecBf16x2) {
             std::vector<uint32_t> w(nv);
-            for (size_t i = 0; i < nv; ++i) w[i] = bf16x2_word(sdat[i]);
+            for (size_t i = 0; i < nv; ++i) {
+                w[i] = bf16x2_word(sdat[i]);
+                bound = std::max(bound, rel_err(sdat[i], bf16x2_decode(w[i])));
+            }
             enc.assign(reinterpret_cast<unsigned char*>(w.data()), reinterpret_cast<unsigned char*>(w.data()) + nv * 4);
         } else if (codec_row_scaled(codec)) {
             scale.assign((size_t)std::max(1, brows), 1.0f);
@@ -952,6 +989,7 @@ int build_bsr_coded(clspmv_ctx_t ctx, clspmv_part_t p, const clspmv_bsr_matrix* 
                     for (unsigned q = 0; q < bs; ++q) {
                         const size_t i = (size_t)q * sstride + (size_t)sp[s] + (size_t)k * S + t;
                         h[i] = row_encode(codec, sdat[i] / scale[(size_t)r]);
+                        bound = std::max(bound, rel_err(sdat[i], (double)scale[(size_t)r] * row_decode(codec, h[i])));
                     }
             }
             enc.assign(reinterpret_cast<unsigned char*>(h.data()), reinterpret_cast<unsigned char*>(h.data()) + nv * 2);
@@ -964,8 +1002,12 @@ int build_bsr_coded(clspmv_ctx_t ctx, clspmv_part_t p, const clspmv_bsr_matrix* 
                     for (int i = sp[s]; i < sp[s + 1]; ++i) amax = std::max(amax, std::fabs(sdat[(size_t)q * sstride + i]));
                     const float sc = fp8_scale(amax);
                     scale[(size_t)q * nsl + s] = sc;
-                    for (int i = sp[s]; i < sp[s + 1]; ++i)
-                        enc[(size_t)q * sstride + i] = f32_to_e4m3(sdat[(size_t)q * sstride + i] / sc);
+                    for (int i = sp[s]; i < sp[s + 1]; ++i) {
+                        const float a = sdat[(size_t)q * sstride + i];
+                        const uint8_t e = f32_to_e4m3(a / sc);
+                        enc[(size_t)q * sstride + i] = e;
+                        bound = std::max(bound, rel_err(a, (double)sc * e4m3_to_f32(e)));
+                    }
                 }
         }
         int rc = 0;
@@ -1010,6 +1052,7 @@ int build_bsr_coded(clspmv_ctx_t ctx, clspmv_part_t p, const clspmv_bsr_matrix* 
             std::snprintf(name, sizeof(name), "bsr%u%u_mixed_%s%s%s", bh, bw, codec_name(codec), delta ? "_d16" : "",
                           d.win ? "_xw" : "");
             fill_info(p, CLSPMV_FMT_BSR, kMethodMixed + codec + (delta ? 8 : 0), rows, cols, m->matinfo.nnz, name, 1);
+            p->info.codec_bound = bound;
         }
         return CLSPMV_OK;
     }
@@ -1445,10 +1488,14 @@ int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int code
         const int nsl = L / 256;  // FP8 slices of 256 rows
         std::vector<float> scale;
         std::vector<unsigned char> enc;
+        double bound = 0.0;
         if (codec == kCodecBf16x2) {
             std::vector<uint32_t> w(n);
-#pragma omp parallel for schedule(static) num_threads(omp_threads())
-            for (long long i = 0; i < (long long)n; ++i) w[(size_t)i] = bf16x2_word(m->dia_data[i]);
+#pragma omp parallel for schedule(static) num_threads(omp_threads()) reduction(max : bound)
+            for (long long i = 0; i < (long long)n; ++i) {
+                w[(size_t)i] = bf16x2_word(m->dia_data[i]);
+                bound = std::max(bound, rel_err(m->dia_data[i], bf16x2_decode(w[(size_t)i])));
+            }
             enc.assign(reinterpret_cast<unsigned char*>(w.data()), reinterpret_cast<unsigned char*>(w.data()) + n * 4);
         } else if (codec_row_scaled(codec)) {
             scale.assign((size_t)L, 1.0f);
@@ -1459,14 +1506,17 @@ int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int code
             }
             if (int e = check_row_scales(codec, scale)) return e;
             std::vector<uint16_t> h(n);
-#pragma omp parallel for schedule(static) num_threads(omp_threads())
-            for (long long i = 0; i < (long long)n; ++i)
-                h[(size_t)i] = row_encode(codec, m->dia_data[i] / scale[(size_t)(i % L)]);
+#pragma omp parallel for schedule(static) num_threads(omp_threads()) reduction(max : bound)
+            for (long long i = 0; i < (long long)n; ++i) {
+                const float sc = scale[(size_t)(i % L)];
+                h[(size_t)i] = row_encode(codec, m->dia_data[i] / sc);
+                bound = std::max(bound, rel_err(m->dia_data[i], (double)sc * row_decode(codec, h[(size_t)i])));
+            }
             enc.assign(reinterpret_cast<unsigned char*>(h.data()), reinterpret_cast<unsigned char*>(h.data()) + n * 2);
         } else {
             scale.assign((size_t)nd * nsl, 1.0f);
             enc.assign(n, 0);
-#pragma omp parallel for schedule(static) num_threads(omp_threads())
+#pragma omp parallel for schedule(static) num_threads(omp_threads()) reduction(max : bound)
             for (long long q = 0; q < (long long)nd * nsl; ++q) {
                 const size_t d = (size_t)q / nsl, s = (size_t)q % nsl;
                 const float* v = m->dia_data + d * L + s * 256;
@@ -1474,7 +1524,11 @@ int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int code
                 for (int i = 0; i < 256; ++i) a = std::max(a, std::fabs(v[i]));
                 const float sc = fp8_scale(a);
                 scale[(size_t)q] = sc;
-                for (int i = 0; i < 256; ++i) enc[d * L + s * 256 + (size_t)i] = f32_to_e4m3(v[i] / sc);
+                for (int i = 0; i < 256; ++i) {
+                    const uint8_t e = f32_to_e4m3(v[i] / sc);
+                    enc[d * L + s * 256 + (size_t)i] = e;
+                    bound = std::max(bound, rel_err(v[i], (double)sc * e4m3_to_f32(e)));
+                }
             }
         }
         int rc = 0;
@@ -1500,6 +1554,7 @@ int clspmv_part_dia_mixed(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int code
         char name[40];
         std::snprintf(name, sizeof(name), "dia_mixed_%s", codec_name(codec));
         fill_info(p, CLSPMV_FMT_DIA, kMethodMixed + codec, rows, cols, m->matinfo.nnz, name, 1);
+        p->info.codec_bound = bound;
         return CLSPMV_OK;
     });
 }
@@ -1529,9 +1584,11 @@ extern "C" int clspmv_cocktail_option(const char* name, long long value);  // co
 
 int clspmv_set_option(const char* name, long long value) {
     if (!name) return CLSPMV_E_INVALID;
-    if (std::strcmp(name, "cocktail_merge_flat") == 0 || std::strcmp(name, "cocktail_split_local") == 0 ||
-        std::strcmp(name, "cocktail_value_codec") == 0)
-        return clspmv_cocktail_option(name, value);
+    if (std::strncmp(name, "cocktail_", 9) == 0) {
+        const int rc = clspmv_cocktail_option(name, value);
+        if (rc) set_error("option %s: unknown or value %lld out of range", name, value);
+        return rc;
+    }
     if (std::strcmp(name, "xtile_max_groups") == 0) {
         if (value < 1 || value > kXtMaxGroups) return CLSPMV_E_INVALID;
         g_xt_max_groups = (int)value;
@@ -1632,6 +1689,51 @@ int clspmv_probe_copy(clspmv_ctx_t ctx, const void* d_src, void* d_dst, size_t b
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     *gbps = (float)(2.0 * (double)bytes * ntimes / (ms * 1e-3) / 1e9);
+    // the copy is checked: every 16-byte word of dst equals src
+    DevBuf cnt;
+    int rc = cnt.alloc(sizeof(unsigned long long));
+    unsigned long long bad = 0;
+    if (!rc) rc = clspmv_memcpy_h2d(ctx, cnt.ptr, &bad, sizeof(bad));
+    if (!rc) rc = hip_status((hipError_t)launch_compare_probe(d_src, d_dst, bytes / 16, (unsigned long long*)cnt.ptr, st),
+                             "compare probe");
+    if (!rc) rc = clspmv_memcpy_d2h(ctx, &bad, cnt.ptr, sizeof(bad));
+    cnt.release();
+    if (!rc && bad) {
+        set_error("copy probe: %llu of %zu 16-byte words differ", bad, bytes / 16);
+        return CLSPMV_E_HIP;
+    }
+    return rc;
+}
+
+// Launch cost of a captured hipGraph of `nodes` empty kernels: microseconds
+// per graph launch over ntimes back-to-back launches.
+int clspmv_probe_graph_launch(clspmv_ctx_t ctx, int nodes, int ntimes, float* usec) {
+    if (!ctx || !usec || nodes <= 0 || nodes > 4096 || ntimes <= 0) return CLSPMV_E_INVALID;
+    hipStream_t cs = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_status(e, "hipStreamCreate");
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int i = 0; i < nodes; ++i) launch_empty(cs);
+        e = hipStreamEndCapture(cs, &graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) return hip_status(e, "graph capture");
+    hipStream_t st = ctx->stream();
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipGraphLaunch(exec, st);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, st);
+    for (int i = 0; i < ntimes && e == hipSuccess; ++i) e = hipGraphLaunch(exec, st);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipGraphExecDestroy(exec);
+    if (e != hipSuccess) return hip_status(e, "graph launch probe");
+    *usec = ms * 1000.f / ntimes;
     return CLSPMV_OK;
 }
 

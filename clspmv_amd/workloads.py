@@ -68,6 +68,34 @@ def best_format(name: str, coo):
     raise KeyError(name)
 
 
+def reference_bytes(name: str, coo) -> int:
+    """Algorithmic bytes of one y += A x in the reference format each c1
+    matrix is named for: the format's stored arrays (values, indices,
+    pointers, its own padding) + 4 B per column (x read once) + 8 B per row
+    (y read and written).  DIA: 4 B per slot of ndiag x align(rows, 32);
+    ELL: 8 B per slot of max-row-length x align(rows, 32); CSR: 8 B per
+    nonzero + 4 B per row pointer; BCSR b x b: 4 b^2 + 4 B per block + 4 B
+    per block-row pointer."""
+    import numpy as np
+
+    rows, cols, nnz = coo.height, coo.width, coo.nnz
+    xy = 4 * cols + 8 * rows
+    al = (rows + 31) // 32 * 32
+    if name in ("c0", "c1a"):
+        ndiag = np.unique(coo.col_id.astype(np.int64) - coo.row_id).size
+        return 4 * ndiag * al + xy
+    if name == "c1b":
+        width = int(np.bincount(coo.row_id, minlength=rows).max()) if nnz else 0
+        return 8 * width * al + xy
+    if name == "c1c":
+        return 8 * nnz + 4 * (rows + 1) + xy
+    if name == "c1d":
+        b = 3
+        keys = np.unique((coo.row_id.astype(np.int64) // b) * ((cols + b - 1) // b) + coo.col_id // b)
+        return (4 * b * b + 4) * keys.size + 4 * ((rows + b - 1) // b + 1) + xy
+    raise KeyError(name)
+
+
 def build(name: str, scale: int = 1) -> Workload:
     coo = make_coo(name, scale)
     fmt, meth, mat, desc = best_format(name, coo)

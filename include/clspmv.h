@@ -295,6 +295,13 @@ typedef struct {
     long long alg_bytes;   /* stored_bytes + 4*cols + 8*rows (x read, y read+write) */
     int    launches;       /* kernels per SpMV */
     char   kernel[64];     /* name of the dominant kernel */
+    /* Mixed-precision parts: the largest relative error of a stored value
+     * against the FP32 value it encodes, max |dec(enc(a)) - a| / |a| over
+     * every nonzero a (a zero stored for a nonzero a counts 1).  It bounds
+     * the componentwise error of y = A x from the encoding, for any x:
+     * |sum_j (dec(a_ij) - a_ij) x_j| <= codec_bound * sum_j |a_ij x_j|.
+     * 0 for FP32 parts. */
+    double codec_bound;
 } clspmv_part_info;
 
 int clspmv_part_csr_scalar(clspmv_ctx_t ctx, const clspmv_csr_matrix* m, int method, clspmv_part_t* p);
@@ -381,7 +388,13 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  *                       part;
  *   "cocktail_value_codec" 0 (default) or a CLSPMV_VAL_* codec for the row-run
  *                       parts that have a mixed-precision variant (x-tiled,
- *                       sliced BSR); the others stay FP32. */
+ *                       sliced BSR); the others stay FP32;
+ *   "cocktail_codec_tol" error guard of that codec, in units of 1e-9
+ *                       (default 100000 = 1e-4; 0 = no guard): a run part
+ *                       whose codec_bound (clspmv_part_info) exceeds it is
+ *                       built in FP32 instead;
+ *   "cocktail_graph"    1 (default): do_spmv and power_iterate replay
+ *                       captured hipGraphs; 0: direct launches. */
 int clspmv_set_option(const char* name, long long value);
 int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info);
 int clspmv_part_free(clspmv_part_t p);
@@ -389,11 +402,17 @@ int clspmv_part_free(clspmv_part_t p);
 /* ------------------------------------------------------------------ */
 /* Bandwidth / launch-overhead probes: bandwidth_test and bench_overhead
  * (A/src/mem_bandwidth.cpp:14-579, A/src/bench_overhead.cpp:14-184). */
+/* Copy kernel of `bytes` (a multiple of 16) run ntimes: read + write GB/s.
+ * The copy is then checked word by word; CLSPMV_E_HIP if dst != src. */
 int clspmv_probe_copy(clspmv_ctx_t ctx, const void* d_src, void* d_dst, size_t bytes, int ntimes, float* gbps);
 int clspmv_probe_launch(clspmv_ctx_t ctx, int ntimes, float* usec_per_launch);
 /* Empty kernel of `groups` x `threads` launched ntimes back to back: the
  * per-launch time of the overhead table (A/src/bench_overhead.cpp:117-165). */
 int clspmv_probe_launch_grid(clspmv_ctx_t ctx, int groups, int threads, int ntimes, float* usec_per_launch);
+/* A captured hipGraph of `nodes` empty kernels launched ntimes back to
+ * back: microseconds per graph launch (the graph column of the overhead
+ * table; the Cocktail driver replays its passes from such graphs). */
+int clspmv_probe_graph_launch(clspmv_ctx_t ctx, int nodes, int ntimes, float* usec_per_launch);
 
 #ifdef __cplusplus
 }

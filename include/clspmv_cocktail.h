@@ -108,8 +108,15 @@ int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* mat, const 
 int clspmv_free_mat_kernels(clspmv_gpumat_t gpumat);
 /* do_spmv (:1088-1132): ntimes * (y += A x), parts in the reference's fixed
  * order BDIA, DIA, SBELL, BELL, BCSR, SELL, ELL, CSR, COO on one stream;
- * asynchronous (clspmv_synchronize waits). */
+ * asynchronous (clspmv_synchronize waits).  The passes are replayed from
+ * hipGraphs captured on first use (up to 64 passes per graph launch);
+ * clspmv_set_option("cocktail_graph", 0) launches each part directly. */
 int clspmv_do_spmv(clspmv_gpumat_t gpumat, int ntimes);
+/* The iteration of the reference's example (A/src/eval.cpp:19-72, with
+ * cpy_vector_from_gpu :1510-1556): `iterations` times { y += A x; x := y },
+ * all on the device, from captured graphs of up to 64 iterations.  Square
+ * matrices only; asynchronous. */
+int clspmv_power_iterate(clspmv_gpumat_t gpumat, int iterations);
 /* cpy_vector_from_cpu / _from_gpu, cpy_result_from_cpu / _from_gpu /
  * _to_cpu / _to_gpu (:1458-1588).  *_gpu take device pointers. */
 int clspmv_cpy_vector_from_cpu(clspmv_gpumat_t gpumat, const float* vec, int vec_size);
@@ -125,6 +132,26 @@ int clspmv_gpumat_buffers(clspmv_gpumat_t gpumat, void** d_vec, void** d_res);
  * nnz and stored bytes (same fields as clspmv_part_info). */
 int clspmv_gpumat_num_parts(clspmv_gpumat_t gpumat);
 int clspmv_gpumat_part_info(clspmv_gpumat_t gpumat, int i, clspmv_part_info* info);
+
+/* evaluate (A/include/spmv_cocktail.h:166, A/src/spmv_cocktail.cpp:1431-1455
+ * with check_and_time :1134-1154): uploads the partition on device 0 with
+ * x = vec and y = res, runs one do_spmv and compares y with coores (the
+ * caller's CPU y = res + A vec, e.g. spmv_only), times 1000 do_spmv and
+ * prints the reference's report (two_vec_compare fields, ms, GFLOPS). */
+int clspmv_evaluate(const clspmv_cocktail* mat, const float* vec, const float* res, const float* coores);
+
+/* The same on a caller's context with ntimes timed runs, the figures
+ * returned (printed only when verbose != 0).  distance, max_diff and
+ * max_diff_ratio are two_vec_compare's (A/src/util.cpp:93-131; sign-blind);
+ * max_rel_err = max_i |coores_i - y_i| / max(|coores_i|, |y_i|) (signed). */
+typedef struct {
+    double distance, max_diff, max_diff_ratio, max_rel_err;
+    int max_diff_id;
+    int ntimes, nparts;
+    double ms_per_spmv, gflops, alg_gbps;
+} clspmv_eval_report;
+int clspmv_evaluate_report(clspmv_ctx_t ctx, const clspmv_cocktail* mat, const float* vec, const float* res,
+                           const float* coores, int ntimes, int verbose, clspmv_eval_report* report);
 
 #ifdef __cplusplus
 }

@@ -29,12 +29,17 @@ int clspmv_mlp_create(clspmv_ctx_t ctx, int in, int hidden, int classes, uint64_
 int clspmv_mlp_free(clspmv_mlp_t m);
 long long clspmv_mlp_num_params(clspmv_mlp_t m);
 int clspmv_mlp_get_params(clspmv_mlp_t m, float* params);
+/* Also resets the Adam moments and step count (a new optimisation). */
 int clspmv_mlp_set_params(clspmv_mlp_t m, const float* params);
 /* Loss (mean cross-entropy) and its gradient with respect to every
  * parameter, for n samples x (n x in) with labels in [0, classes). */
 int clspmv_mlp_loss_grad(clspmv_mlp_t m, const float* x, const int* labels, int n, float* loss, float* grads);
 /* `steps` full-batch Adam steps (lr, beta1, beta2, eps, weight decay 0);
- * writes the final loss. */
+ * writes the final loss.  The update is Kingma & Ba's Algorithm 1:
+ * m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+ * p -= lr (m / (1-b1^t)) / (sqrt(v / (1-b2^t)) + eps).  The moments and the
+ * step number t persist in the handle, so k calls of s steps take the same
+ * steps as one call of k*s. */
 int clspmv_mlp_train(clspmv_mlp_t m, const float* x, const int* labels, int n, int steps, float lr, float beta1,
                      float beta2, float eps, float* loss);
 /* Logits (n x classes) and argmax class of each sample. */

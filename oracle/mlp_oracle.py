@@ -4,9 +4,13 @@ library).  Same parameter layout as include/clspmv_mlp.h: W1 (in x hidden),
 b1, W2 (hidden x hidden), b2, W3 (hidden x classes), b3; ReLU hidden layers,
 softmax cross-entropy averaged over the samples, Adam.
 
-Pinned by: central finite differences of the loss (gradients), and the
+Pinned by: central finite differences of the loss (gradients), the
 closed form of the softmax cross-entropy gradient at zero weights
-(p = 1/C for every class)."""
+(p = 1/C for every class), and, for the Adam step, closed forms of
+Kingma & Ba's Algorithm 1 (arXiv:1412.6980): with a constant gradient g
+the bias-corrected moments are exactly g and g^2 at every step, so each
+step moves a parameter by lr * g / (|g| + eps) (about lr * sign(g)); a
+step with g = 0 from zero moments leaves it unchanged."""
 import numpy as np
 
 
@@ -59,3 +63,15 @@ def loss_only(p, x, y, d, h, c):
     z = forward(p, x, d, h, c)[2]
     zm = z - z.max(axis=1, keepdims=True)
     return float(np.mean(np.log(np.exp(zm).sum(axis=1)) - zm[np.arange(n), y]))
+
+
+def adam_step(p, g, m, v, t, lr, b1, b2, eps):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1) in FP64: returns the
+    new (p, m, v) for step number t (1-based)."""
+    p = np.asarray(p, np.float64)
+    g = np.asarray(g, np.float64)
+    m = b1 * np.asarray(m, np.float64) + (1.0 - b1) * g
+    v = b2 * np.asarray(v, np.float64) + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** t)
+    vh = v / (1.0 - b2 ** t)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -172,3 +172,97 @@ def test_power_iteration(ctx, tables):
     scale = np.maximum(np.abs(yc), 1e-3 * np.abs(yc).max())
     assert np.max(np.abs(y - yc) / scale) < 1e-4
     gpu.free()
+
+
+def test_power_iteration_device_equals_host_loop(ctx, tables):
+    # the device-resident iteration (captured graphs, x := y by a kernel)
+    # takes bitwise the same steps as the reference's host-driven loop
+    # (do_spmv, then cpy_vector_from_gpu), also across graph sizes (70
+    # iterations = a 64-iteration graph and a 6-iteration graph)
+    coo = AC.make(GEN["cocktail_64k"])
+    vals = (coo.data * np.float32(0.05)).astype(np.float32)
+    coo = M.CooMatrix.from_arrays(coo.row_id, coo.col_id, vals, coo.width, coo.height)
+    mat = CK.analyze_matrix(coo.copy(), tables)
+    x = M.gen_vector(coo.width, 31)
+    y0 = np.zeros(coo.height, np.float32)
+    out = {}
+    for k in (3, 70):
+        a = CK.GpuCocktail(ctx, mat, x, y0)
+        b = CK.GpuCocktail(ctx, mat, x, y0)
+        out["dev", k] = CK.power_iterate(ctx, a, k)
+        out["host", k] = CK.power_iterate_host(ctx, b, k)
+        assert np.array_equal(out["dev", k].view(np.uint32), out["host", k].view(np.uint32)), k
+        a.free()
+        b.free()
+    # the first step is y = 0 + A x: the oracle's product
+    g = CK.GpuCocktail(ctx, mat, x, y0)
+    rep = O.parity_report(CK.power_iterate(ctx, g, 1), coo, x, y0)
+    assert rep["ok"], rep
+    g.free()
+
+
+def test_graph_replays_equal_direct_launches(ctx, tables):
+    # do_spmv replayed from captured graphs equals direct launches bitwise,
+    # replay after replay, with eager copies and host synchronisations in
+    # between (each replay checked)
+    coo = AC.make(GEN["band15_rand8"])
+    mat = CK.analyze_matrix(coo.copy(), tables)
+    x = M.gen_vector(coo.width, 33)
+    y0 = M.gen_vector(coo.height, 34)
+    g = CK.GpuCocktail(ctx, mat, x, y0)
+    assert len(g.part_infos()) >= 2
+    ref = []
+    L.check(L.lib().clspmv_set_option(b"cocktail_graph", 0), "set_option")
+    try:
+        for n in (1, 2, 65):
+            g.result_from_cpu(y0)
+            g.do_spmv(n)
+            ref.append(g.result_to_cpu())
+    finally:
+        L.check(L.lib().clspmv_set_option(b"cocktail_graph", 1), "set_option")
+    for rep in range(3):
+        for n, r in zip((1, 2, 65), ref):
+            g.result_from_cpu(y0)
+            g.do_spmv(n)
+            ctx.sync()
+            assert np.array_equal(g.result_to_cpu().view(np.uint32), r.view(np.uint32)), (rep, n)
+    rep1 = O.parity_report(ref[0], coo, x, y0)
+    assert rep1["ok"], rep1
+    g.free()
+
+
+def test_evaluate_report(ctx, tables):
+    # evaluate: one checked do_spmv against the caller's CPU result, then a
+    # timed loop; the reference's two_vec_compare fields are reported
+    coo = AC.make(GEN["band15_rand8"])
+    mat = CK.analyze_matrix(coo.copy(), tables)
+    x = M.gen_vector(coo.width, 35)
+    res = M.gen_vector(coo.height, 36)
+    coores = (O.spmv_only(coo.row_id, coo.col_id, coo.data, x, coo.height) + res).astype(np.float32)
+    r = CK.evaluate(ctx, mat, x, res, coores, ntimes=20)
+    assert r["ntimes"] == 20 and r["nparts"] >= 1 and r["ms_per_spmv"] > 0 and r["gflops"] > 0
+    assert r["max_rel_err"] < 1e-4 and r["max_diff_ratio"] <= r["max_rel_err"] + 1e-12
+    # a wrong expected vector is reported, not hidden
+    bad = coores.copy()
+    bad[7] += 1.0
+    r2 = CK.evaluate(ctx, mat, x, res, bad, ntimes=1)
+    assert r2["max_diff_id"] == 7 and r2["max_diff"] > 0.5
+
+
+def test_probes(ctx):
+    # the copy probe checks what it copied; graph launches are timed
+    import ctypes as C
+
+    n = 1 << 22
+    src = M.DeviceArray.from_numpy(ctx, M.gen_vector(n, 37))
+    dst = M.DeviceArray(ctx, n)
+    g = C.c_float()
+    L.check(L.lib().clspmv_probe_copy(ctx.h, src.ptr, dst.ptr, 4 * n, 3, C.byref(g)), "probe_copy")
+    assert g.value > 100
+    assert np.array_equal(dst.download(), src.download())
+    for nodes in (1, 16):
+        us = C.c_float()
+        L.check(L.lib().clspmv_probe_graph_launch(ctx.h, nodes, 20, C.byref(us)), "probe_graph_launch")
+        assert 0 < us.value < 10000
+    src.free()
+    dst.free()

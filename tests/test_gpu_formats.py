@@ -241,6 +241,59 @@ def test_bsr_sliced_int16_steps_and_fallback(ctx):
         _check(coo, y, x, y0, f"BSR int16 steps {info['kernel']}")
 
 
+def _blocks_coo(blocks, brows, cols, b=2, seed=0):
+    """COO of dense b x b blocks {block row: [first column, ...]}."""
+    rng = np.random.default_rng(seed)
+    r, c = [], []
+    for br, firsts in blocks.items():
+        for c0 in firsts:
+            for i in range(b):
+                for j in range(b):
+                    r.append(br * b + i)
+                    c.append(c0 + j)
+    r, c = np.array(r, np.int32), np.array(c, np.int32)
+    o = np.lexsort((c, r))
+    v = rng.standard_normal(r.size).astype(np.float32)
+    return M.CooMatrix.from_arrays(r[o], c[o], v, cols, brows * b)
+
+
+def _swap_blocks(bsr, a, b):
+    """Swap blocks a and b of the BSR arrays (same matrix, other order)."""
+    col, dat, st = bsr.col_id, bsr.data, bsr.s.bsr_block_stride
+    col[a], col[b] = col[b], col[a]
+    for q in range(bsr.s.bsr_bwidth * bsr.s.bsr_bheight):
+        dat[q * st + a], dat[q * st + b] = dat[q * st + b], dat[q * st + a]
+
+
+# (name, blocks per block row, block rows, swap the first two blocks of block
+# row 0, int16 steps expected): steps of +-32768 are the int16 limits (2x2
+# block columns are even, so +32766 is the largest positive step)
+BSR_STEP_CASES = [
+    ("step_32766", {0: [0, 32766], 1: [2, 4]}, 2, False, True),
+    ("step_32768", {0: [0, 32768], 1: [2, 4]}, 2, False, False),
+    ("step_minus_32768", {0: [7232, 40000], 1: [2, 4]}, 2, True, True),
+    ("step_minus_32770", {0: [7230, 40000], 1: [2, 4]}, 2, True, False),
+    ("empty_slice", {**{r: [2 * r, 2 * r + 30000] for r in range(64)},
+                     **{r: [2 * r - 256, 2 * r + 32000] for r in range(128, 200)}}, 200, False, True),
+]
+
+
+@pytest.mark.parametrize("name,blocks,brows,swap,d16", BSR_STEP_CASES, ids=[c[0] for c in BSR_STEP_CASES])
+def test_bsr_sliced_int16_step_limits(ctx, name, blocks, brows, swap, d16):
+    # int16 column steps hold exactly the steps in [-32768, 32767]; a step
+    # outside keeps int32 columns; a block row stored out of column order
+    # (negative steps) and a slice of 64 empty block rows are handled
+    coo = _blocks_coo(blocks, brows, 70000, seed=len(name))
+    bsr = M.coo2bsr(coo, 2, 2)
+    if swap:
+        _swap_blocks(bsr, 0, 1)
+        assert bsr.col_id[0] > bsr.col_id[1]
+    x, y0 = _x_y(coo)
+    y, info = gpu_spmv(ctx, L.FMT_BSR, bsr, x, y0, 0)
+    assert info["kernel"].startswith("bsr22_sliced") and ("_d16" in info["kernel"]) == d16, (name, info)
+    _check(coo, y, x, y0, f"BSR int16 step limits {name} {info['kernel']}")
+
+
 @pytest.mark.parametrize("lanes", [2, 4, 8, 16])
 def test_bsr_explicit_lanes(ctx, lanes):
     # method L = lanes per block row (16 shares its number with no other kind)

@@ -283,10 +283,12 @@ def test_cocktail_value_codec(ctx, codec):
     mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(CK.REFERENCE_TABLES))
     x, y0 = M.gen_vector(coo.width, 29), M.gen_vector(coo.height, 30)
     L.check(L.lib().clspmv_set_option(b"cocktail_value_codec", C.c_longlong(codec)), "set_option")
+    L.check(L.lib().clspmv_set_option(b"cocktail_codec_tol", C.c_longlong(0)), "set_option")  # no guard
     try:
         gpu = CK.GpuCocktail(ctx, mat, x, y0)
     finally:
         L.check(L.lib().clspmv_set_option(b"cocktail_value_codec", C.c_longlong(0)), "set_option")
+        L.check(L.lib().clspmv_set_option(b"cocktail_codec_tol", C.c_longlong(100000)), "set_option")
     names = [p["kernel"] for p in gpu.part_infos()]
     assert names[0].startswith("bsr44_mixed_") and all("mixed" in n or n.startswith(("xtile_", "csr_sliced"))
                                                        for n in names), names
@@ -294,3 +296,76 @@ def test_cocktail_value_codec(ctx, codec):
     _check(codec, gpu.result_to_cpu(), coo, x, y0, window=3)
     gpu.free()
 
+
+
+MIXED_CASES = ([(L.FMT_DIA, n, mk) for n, mk in DIA_CASES[:3]] + [(L.FMT_BSR, n, mk) for n, mk in BSR_CASES[1:3]]
+               + [(L.FMT_CSR_VECTOR, n, mk) for n, mk in CSR_CASES[:3]])
+
+
+@pytest.mark.parametrize("codec", CODECS)
+@pytest.mark.parametrize("fmt,name,make", MIXED_CASES, ids=[f"{c[0]}_{c[1]}" for c in MIXED_CASES])
+def test_codec_bound_bounds_the_error(ctx, codec, fmt, name, make):
+    # the encoder's codec_bound (largest relative error of a stored value)
+    # bounds the componentwise error of y = A x for any x, up to the FP32
+    # arithmetic (1e-5 of S); BF16 hi+lo holds ~16 bits, FP16 ~11 bits
+    coo = make()
+    mat = M.coo2dia(coo, 256) if fmt == L.FMT_DIA else M.coo2bsr(coo, 3, 3) if fmt == L.FMT_BSR else M.coo2csr(coo)
+    part = M.MixedPart(ctx, fmt, mat, codec)
+    bound = part.info["codec_bound"]
+    assert 0 <= bound <= 1
+    if codec == L.VAL_BF16X2:
+        assert bound <= 2.0 ** -15, bound
+    if codec == L.VAL_FP16_ROW:
+        assert bound <= 2.0 ** -10, bound
+    y0 = np.zeros(coo.height, np.float32)
+    for x in (M.gen_vector(coo.width, 1), np.ones(coo.width, np.float32)):
+        e = _err(_run(ctx, part, coo, x, y0), coo, x, y0)
+        assert e <= bound + 1e-5, (e, bound)
+    part.free()
+
+
+def _with_values(coo, vals):
+    return M.CooMatrix.from_arrays(coo.row_id.copy(), coo.col_id.copy(), np.asarray(vals, np.float32), coo.width,
+                                   coo.height)
+
+
+@pytest.mark.parametrize("codec", CODECS)
+def test_cocktail_codec_guard(ctx, codec):
+    # The Cocktail driver keeps a run's codec only when its codec_bound is
+    # within 1e-4 (the default guard).  N(0,1) values: only BF16 hi+lo keeps
+    # 1e-4 for every x; values in {+-1, 2, 4}: every codec holds them within
+    # 1e-4; the same with one planted value 1e-6 in a scattered row: that run
+    # falls back to FP32 except under BF16, the block run keeps its codec.
+    import ctypes as C
+
+    from clspmv_amd import cocktail as CK
+    from tests.test_gpu_cocktail import _blocks_scattered
+
+    base = _blocks_scattered(41)
+    rng = np.random.default_rng(5)
+    ints = _with_values(base, rng.choice([-4.0, -2.0, -1.0, 1.0, 2.0, 4.0], base.nnz))
+    planted_vals = ints.data.copy()
+    k = int(np.searchsorted(base.row_id, 40000))
+    planted_vals[k] *= 1e-6
+    planted = _with_values(base, planted_vals)
+    tables = CK.BenchTables(CK.REFERENCE_TABLES)
+    bf16 = codec == L.VAL_BF16X2
+    for coo, expect in ((base, [bf16, bf16]), (ints, [True, True]), (planted, [True, bf16])):
+        mat = CK.analyze_matrix(coo.copy(), tables)
+        x, y0 = M.gen_vector(coo.width, 29), M.gen_vector(coo.height, 30)
+        L.check(L.lib().clspmv_set_option(b"cocktail_value_codec", C.c_longlong(codec)), "set_option")
+        try:
+            gpu = CK.GpuCocktail(ctx, mat, x, y0)
+        finally:
+            L.check(L.lib().clspmv_set_option(b"cocktail_value_codec", C.c_longlong(0)), "set_option")
+        infos = gpu.part_infos()
+        kept = ["mixed" in p["kernel"] or "_" + {1: "bf16x2", 2: "fp16row", 3: "fp8slice", 4: "i16row"}[codec]
+                in p["kernel"] for p in infos]
+        assert kept == expect, ([p["kernel"] for p in infos], expect)
+        for p, kp in zip(infos, kept):
+            assert (p["codec_bound"] <= 1e-4) if kp else p["codec_bound"] == 0.0, p
+        gpu.do_spmv(1)
+        e = _err(gpu.result_to_cpu(), coo, x, y0)
+        assert e <= 1e-4 + 1e-5, ([p["kernel"] for p in infos], e)
+        gpu.free()
+        mat.free()

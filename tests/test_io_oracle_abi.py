@@ -141,6 +141,110 @@ def test_per_format_oracles_match_dense():
     assert np.allclose(y, ref, atol=1e-4)
 
 
+def _dense(case):
+    h, w = (int(v) for v in CONV[f"{case}/shape"])
+    A = np.zeros((h, w), np.float64)
+    np.add.at(A, (CONV[f"{case}/in_row"], CONV[f"{case}/in_col"]), CONV[f"{case}/in_val"].astype(np.float64))
+    return A
+
+
+def _dia_oracles(offsets, data, L_, bptr, boffsets, x, y0, w, h):
+    """oracle_dia_spmv and oracle_bdia_spmv (both y += A x) on one layout."""
+    o = O.lib()
+    P = lambda a, t=C.c_int: np.ascontiguousarray(a).ctypes.data_as(C.POINTER(t))  # noqa: E731
+    F = C.c_float
+    yd = np.array(y0, np.float32)
+    o.oracle_dia_spmv(len(offsets), L_, P(offsets), P(data, F), P(x, F), w, h, P(yd, F))
+    yb = np.array(y0, np.float32)
+    o.oracle_bdia_spmv(len(boffsets), L_, P(bptr), P(boffsets), P(data, F), P(x, F), w, h, P(yb, F))
+    return yd, yb
+
+
+def test_dia_bdia_oracles_on_reference_layouts():
+    """oracle_dia_spmv / oracle_bdia_spmv (A/include/spmv_serial.h:11-34,
+    :59-90) on the DIA and BDIA arrays the reference's own coo2dia /
+    coo2bdia wrote (golden lap2d_12): the 5-point Laplacian's closed form
+    with x = 1 (interior 0, edge 1, corner 2, exactly), and y0 + A x against
+    the dense FP64 product for random x and y0."""
+    case = "lap2d_12"
+    h, w = (int(v) for v in CONV[f"{case}/shape"])
+    _, _, L_ = (int(v) for v in CONV[f"{case}/dia_meta"])
+    offs, data = CONV[f"{case}/dia_offsets"], CONV[f"{case}/dia_data"]
+    bptr, boffs = CONV[f"{case}/bdia_bptr"], CONV[f"{case}/bdia_offsets"]
+    yd, yb = _dia_oracles(offs, data, L_, bptr, boffs, np.ones(w, np.float32), np.zeros(h, np.float32), w, h)
+    g = int(round(h ** 0.5))
+    i, j = np.divmod(np.arange(h), g)
+    expect = (i == 0).astype(np.float32) + (i == g - 1) + (j == 0) + (j == g - 1)
+    assert np.array_equal(yd, expect) and np.array_equal(yb, expect)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, w).astype(np.float32)
+    y0 = rng.uniform(-1, 1, h).astype(np.float32)
+    yd, yb = _dia_oracles(offs, data, L_, bptr, boffs, x, y0, w, h)
+    ref = y0.astype(np.float64) + _dense(case) @ x.astype(np.float64)
+    assert np.allclose(yd, ref, rtol=0, atol=1e-5) and np.allclose(yb, ref, rtol=0, atol=1e-5)
+
+
+@pytest.mark.parametrize("h,w", [(50, 64), (64, 50), (97, 97)])
+def test_dia_bdia_oracles_nonsymmetric(h, w):
+    """The same oracles on non-symmetric, square and rectangular banded
+    matrices (diagonals -7, -2, -1, 0, 3, 4, 5 with random gaps) laid out by
+    the library's coo2dia / coo2bdia (their arrays are pinned to the
+    reference's by the golden conversion tests): a swapped offset sign
+    (A^T x) or a row/column slip fails the comparison, and integer values
+    with x in {-2..2} make the dense product exact."""
+    rng = np.random.default_rng(h * 1000 + w)
+    A = np.zeros((h, w))
+    for off in (-7, -2, -1, 0, 3, 4, 5):
+        for r in range(h):
+            c = r + off
+            if 0 <= c < w and rng.random() < 0.9:
+                A[r, c] = rng.choice([-3, -2, -1, 1, 2, 3])
+    rows, cols = np.nonzero(A)
+    vals = A[rows, cols].astype(np.float32)
+    coo = M.CooMatrix.from_arrays(rows.astype(np.int32), cols.astype(np.int32), vals, w, h)
+    d = M.coo2dia(coo)
+    b = M.coo2bdia(coo)
+    assert list(b.offsets) == [-7, -2, 3] and list(b.bptr) == [0, 1, 4, 7]
+    n = min(h, w)
+    assert not np.array_equal(A[:n, :n], A[:n, :n].T)
+    x = rng.integers(-2, 3, w).astype(np.float32)
+    y0 = rng.integers(-5, 6, h).astype(np.float32)
+    yd, yb = _dia_oracles(d.offsets, d.data, d.s.dia_length_aligned, b.bptr, b.offsets, x, y0, w, h)
+    ref = y0 + A @ x
+    assert np.array_equal(yd, ref) and np.array_equal(yb, ref)
+
+
+def test_coo_spmv_oracle():
+    """oracle_coo_spmv (y += A x in COO order, A/include/spmv_serial.h:184-196):
+    from y = 0 bitwise equal to oracle_spmv_only (itself bit-exact against the
+    reference's compiled spmv_only); from a nonzero y exact on a
+    non-square integer-valued matrix (a transposed operand or a slipped
+    index fails), and within FP32 of the dense FP64 product on real values."""
+    o = O.lib()
+    P = lambda a, t=C.c_int: np.ascontiguousarray(a).ctypes.data_as(C.POINTER(t))  # noqa: E731
+    F = C.c_float
+    coo = M.gen_zipf_rows(3000, 1.8, 8.0, 500, 9)
+    x = M.gen_vector(coo.width, 4)
+    y = np.zeros(coo.height, np.float32)
+    o.oracle_coo_spmv(coo.nnz, P(coo.row_id), P(coo.col_id), P(coo.data, F), P(x, F), P(y, F))
+    ys = O.spmv_only(coo.row_id, coo.col_id, coo.data, x, coo.height)
+    assert np.array_equal(y.view(np.uint32), ys.view(np.uint32))
+    rng = np.random.default_rng(8)
+    h, w = 37, 53
+    A = np.where(rng.random((h, w)) < 0.2, rng.integers(-4, 5, (h, w)), 0).astype(np.float32)
+    r, c = np.nonzero(A)
+    xi = rng.integers(-3, 4, w).astype(np.float32)
+    y0 = rng.integers(-9, 10, h).astype(np.float32)
+    y = y0.copy()
+    o.oracle_coo_spmv(len(r), P(r.astype(np.int32)), P(c.astype(np.int32)), P(A[r, c], F), P(xi, F), P(y, F))
+    assert np.array_equal(y, y0 + A.astype(np.float64) @ xi)
+    xr = rng.uniform(-1, 1, w).astype(np.float32)
+    Ar = np.where(A != 0, rng.standard_normal((h, w)), 0).astype(np.float32)
+    y = y0.copy()
+    o.oracle_coo_spmv(len(r), P(r.astype(np.int32)), P(c.astype(np.int32)), P(Ar[r, c], F), P(xr, F), P(y, F))
+    assert np.allclose(y, y0 + Ar.astype(np.float64) @ xr, rtol=0, atol=1e-5)
+
+
 def test_project_omp_csr_matches_serial_and_f64():
     # the project CPU baseline (OpenMP over rows): each row summed in order,
     # so bitwise equal to the serial CSR oracle; within FP32 of the FP64 product
