@@ -450,9 +450,11 @@ def batched_c4(ctx, args):
             f"({rec['batched_vs_plain']}x)")
     out["by_k"] = by_k
     sel = S.Selector.load_default(ctx)
+    choice = {}
 
     def pick(coo, csr):
         fmt, meth, name = sel.choose(coo)
+        choice[id(coo)] = name
         key = S.CANDIDATES[S.CANDIDATE_NAMES.index(name)][3]
         try:
             mat = S._convert(key, coo) if key != "csr" else csr
@@ -468,6 +470,65 @@ def batched_c4(ctx, args):
                        "kernel": "learned selector's format, k launches per matrix"}
     out["batched_vs_plain"] = round(ms_p / ms_b, 3)
     out["selector_vs_plain"] = round(ms_p / ms_s, 3)
+
+    # the combined form: one graph over all matrices, each matrix in the
+    # path the learned selector picks -- BSR b x b blocks on the matrix
+    # cores where it picks a sliced BSR class and k >= 4, CSR items otherwise
+    # (value codecs: none passes the 1e-4 guard on these N(0,1) values)
+    def combined_entries(idx):
+        mats_c, held = [], []
+        for i in idx:
+            _, coo, k = mats[i]
+            name = choice.get(id(coo), "csr_adaptive")
+            if name.startswith("bsr") and k >= 4:
+                bd = int(name[3])
+                m = M.coo2bsr(coo, bd, bd)
+                mats_c.append(m)
+                held.append(m)
+            else:
+                mats_c.append(csrs[i])
+        return mats_c, held
+
+    def time_batch(bt, reps_=reps):
+        bt.capture()
+        bt.run(1)
+        ctx.sync()
+        ctx.time_begin()
+        bt.run(reps_)
+        return ctx.time_end() / reps_
+
+    all_idx = list(range(len(mats)))
+    mats_c, held = combined_entries(all_idx)
+    bc = B.Batch(ctx, mats_c, ks)
+    for i, (_, coo, k) in enumerate(mats):
+        bc.set_x(i, np.ones((coo.width, k), np.float32))
+    ms_cb = time_batch(bc)
+    st_c = bc.stats()
+    bc.free()
+    n_block = sum(1 for i in all_idx if choice.get(id(mats[i][1]), "").startswith("bsr") and ks[i] >= 4)
+    out["combined"] = {"ms": round(ms_cb, 4), "eff_gnnz_per_s": round(nnz_rhs / (ms_cb * 1e-3) / 1e9, 1),
+                       "alg_gbps": round(st_c["alg_bytes"] / (ms_cb * 1e-3) / 1e9, 1),
+                       "matrices_on_matrix_cores": n_block, "vs_plain": round(ms_p / ms_cb, 3),
+                       "vs_batched_csr": round(ms_b / ms_cb, 3),
+                       "launch": "one hipGraph replay; per matrix the selector's path (BSR MFMA or CSR items)"}
+    for kk in (1, 4, 16):
+        idx = [i for i in all_idx if ks[i] == kk]
+        if not idx:
+            continue
+        mk, hk = combined_entries(idx)
+        bk = B.Batch(ctx, mk, [kk] * len(idx))
+        for j, i in enumerate(idx):
+            bk.set_x(j, np.ones((mats[i][1].width, kk), np.float32))
+        ms_k = time_batch(bk)
+        bk.free()
+        for m in hk:
+            m.free()
+        nz = sum(mats[i][1].nnz for i in idx)
+        by_k[str(kk)]["combined_ms"] = round(ms_k, 4)
+        by_k[str(kk)]["combined_eff_gnnz_per_s"] = round(nz * kk / (ms_k * 1e-3) / 1e9, 1)
+        by_k[str(kk)]["combined_vs_plain"] = round(by_k[str(kk)]["plain_ms"] / ms_k, 3)
+    log(f"c4 combined (selector's path per matrix, {n_block} on the matrix cores): {ms_cb:.2f} ms "
+        f"({out['combined']['vs_plain']}x plain, {out['combined']['vs_batched_csr']}x the CSR batch)")
 
     # plain FP32 Cocktail per matrix (analyzer with the MI355X tables, k
     # do_spmv per matrix) on an eighth of the matrices, against one graph
@@ -506,21 +567,112 @@ def batched_c4(ctx, args):
     bs.run(reps)
     ms_bs = ctx.time_end() / reps
     bs.free()
+    # the combined form on the same matrices, and its error: every right-hand
+    # side of a seeded sixteenth of them against the FP64 product
+    mats_s, held_s = combined_entries(sub)
+    bcs = B.Batch(ctx, mats_s, [ks[i] for i in sub])
+    rng = np.random.default_rng(9)
+    xs = {}
+    for j, i in enumerate(sub):
+        xs[j] = rng.uniform(-1, 1, (mats[i][1].width, ks[i])).astype(np.float32)
+        bcs.set_x(j, xs[j])
+    ms_sub_c = time_batch(bcs)
+    for j in range(len(sub)):
+        bcs.set_y(j, np.zeros((mats[sub[j]][1].height, ks[sub[j]]), np.float32))
+    bcs.run(1)
+    ctx.sync()
+    worst = 0.0
+    checked = 0
+    for j in range(0, len(sub), 8):
+        coo = mats[sub[j]][1]
+        y = bcs.get_y(j)
+        for r in range(ks[sub[j]]):
+            y64, srow = _fp64_reference(coo, xs[j][:, r])
+            worst = max(worst, _errors(y[:, r], y64, srow)[0])
+            checked += 1
+    bcs.free()
+    for m in held_s:
+        m.free()
     nz_sub = sum(mats[i][1].nnz * ks[i] for i in sub)
     out["cocktail_subset"] = {
         "matrices": len(sub), "nnz_x_rhs": int(nz_sub), "analysis_s": round(t_an, 1),
         "cocktail_ms": round(ms_c, 4), "cocktail_eff_gnnz_per_s": round(nz_sub / (ms_c * 1e-3) / 1e9, 1),
         "batched_ms": round(ms_bs, 4), "batched_eff_gnnz_per_s": round(nz_sub / (ms_bs * 1e-3) / 1e9, 1),
         "batched_vs_cocktail": round(ms_c / ms_bs, 3),
+        "combined_ms": round(ms_sub_c, 4), "combined_eff_gnnz_per_s": round(nz_sub / (ms_sub_c * 1e-3) / 1e9, 1),
+        "combined_vs_cocktail": round(ms_c / ms_sub_c, 3),
+        "combined_max_componentwise_err": worst, "combined_err_vectors_checked": checked,
         "kernel": "Cocktail partition per matrix (MI355X tables), k do_spmv each"}
     log(f"c4 Cocktail per matrix ({len(sub)} matrices, analysis {t_an:.1f} s): {ms_c:.2f} ms, batched "
         f"{ms_bs:.2f} ms ({out['cocktail_subset']['batched_vs_cocktail']}x)")
+    for m in held:
+        m.free()
     for csr in csrs:
         csr.free()
     for _, coo, _ in mats:
         coo.free()
     log(f"c4: plain {ms_p:.2f} ms, selector {ms_s:.2f} ms, batched {ms_b:.2f} ms "
         f"(batched {out['batched']['eff_gnnz_per_s']} G nnz*rhs/s, {out['batched_vs_plain']}x plain)")
+    return out
+
+
+def launch_c0(ctx):
+    """Launch overhead and the hot loop on a small matrix (the 256 x 256
+    5-point Laplacian, Cocktail partition with the reference's tables):
+    the launch-overhead table (plain launch, captured graph of n empty
+    kernels), do_spmv from captured graphs against direct launches, and the
+    iteration {y += A x; x := y} on the device against the reference's
+    host-driven loop (do_spmv + cpy_vector_from_gpu per step)."""
+    import ctypes as C
+
+    import numpy as np
+
+    from clspmv_amd import _lib as L
+    from clspmv_amd import cocktail as CK
+    from clspmv_amd import matrix as M
+    from clspmv_amd import workloads as W
+
+    out = {}
+    us = C.c_float()
+    L.check(L.lib().clspmv_probe_launch(ctx.h, 2000, C.byref(us)), "probe_launch")
+    out["plain_launch_us"] = round(us.value, 3)
+    for n in (1, 8, 64):
+        L.check(L.lib().clspmv_probe_graph_launch(ctx.h, n, 500, C.byref(us)), "probe_graph_launch")
+        out[f"graph_{n}_nodes_us_per_launch"] = round(us.value, 3)
+        out[f"graph_{n}_nodes_us_per_node"] = round(us.value / n, 3)
+    c0 = W.make_coo("c0")
+    # values scaled by 1/20 so that 100 steps of y <- (I + A) y stay finite
+    coo = M.CooMatrix.from_arrays(c0.row_id.copy(), c0.col_id.copy(), c0.data * np.float32(0.05), c0.width,
+                                  c0.height)
+    c0.free()
+    mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(CK.REFERENCE_TABLES))
+    x = M.gen_vector(coo.width, 7)
+    gpu = CK.GpuCocktail(ctx, mat, x, np.zeros(coo.height, np.float32))
+    out["parts"] = [p["kernel"] for p in gpu.part_infos()]
+    n = 1000
+    for flag, label in ((1, "do_spmv_graph_us"), (0, "do_spmv_direct_us")):
+        L.check(L.lib().clspmv_set_option(b"cocktail_graph", flag), "set_option")
+        gpu.do_spmv(64)
+        ctx.sync()
+        ctx.time_begin()
+        gpu.do_spmv(n)
+        out[label] = round(ctx.time_end() / n * 1e3, 3)
+    L.check(L.lib().clspmv_set_option(b"cocktail_graph", 1), "set_option")
+    k = 100
+    gpu.result_from_cpu(np.zeros(coo.height, np.float32))
+    gpu.vector_from_cpu(x)
+    t0 = time.perf_counter()
+    CK.power_iterate(ctx, gpu, k)
+    out["iterate_device_us_per_step"] = round((time.perf_counter() - t0) / k * 1e6, 2)
+    gpu.result_from_cpu(np.zeros(coo.height, np.float32))
+    gpu.vector_from_cpu(x)
+    t0 = time.perf_counter()
+    CK.power_iterate_host(ctx, gpu, k)
+    out["iterate_host_loop_us_per_step"] = round((time.perf_counter() - t0) / k * 1e6, 2)
+    gpu.free()
+    mat.free()
+    coo.free()
+    log(f"launch: {out}")
     return out
 
 
@@ -680,6 +832,7 @@ def main():
     c3 = None if args.no_c3 else mixed_c3(ctx, args, work, per)
     c2 = None if args.no_c2 else cocktail_c2(ctx, args)
     c4 = batched_c4(ctx, args) if args.c4_matrices > 0 else None
+    launch = launch_c0(ctx)
 
     cpu = None
     if not args.no_cpu:
@@ -729,6 +882,7 @@ def main():
         "mixed_c3": c3,
         "cocktail_c2": c2,
         "batched_c4": c4,
+        "launch_c0": launch,
     }
     print(json.dumps(out), flush=True)
 

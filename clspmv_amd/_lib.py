@@ -86,6 +86,15 @@ class Bsr(C.Structure):
                 ("bsr_row_ptr", _IP), ("bsr_col_id", _IP), ("bsr_data", _FP)]
 
 
+class PlanPart(C.Structure):
+    _fields_ = [("source", C.c_int), ("format", C.c_int), ("method", C.c_int), ("row0", C.c_int), ("row1", C.c_int),
+                ("nnz", C.c_longlong)]
+
+
+class BatchEntry(C.Structure):
+    _fields_ = [("csr", C.c_void_p), ("bsr", C.c_void_p), ("k", C.c_int)]
+
+
 class SpmmTile(C.Structure):
     _fields_ = [("rows_cap", C.c_int), ("nnz_cap", C.c_int), ("long_threshold", C.c_int)]
 
@@ -193,6 +202,13 @@ _SIGS = [
     ("clspmv_init_mat_kernels", C.c_int, [C.c_void_p, _P(Cocktail), _FP, _FP, C.c_int, _P(C.c_void_p)]),
     ("clspmv_free_mat_kernels", C.c_int, [C.c_void_p]),
     ("clspmv_do_spmv", C.c_int, [C.c_void_p, C.c_int]),
+    ("clspmv_analyze_plan", C.c_int, [C.c_void_p, _P(Coo), C.c_float, C.c_int, C.c_int, _P(C.c_void_p)]),
+    ("clspmv_plan_cocktail", C.c_int, [C.c_void_p, _P(C.c_void_p)]),
+    ("clspmv_plan_num_parts", C.c_int, [C.c_void_p]),
+    ("clspmv_plan_get_part", C.c_int, [C.c_void_p, C.c_int, _P(PlanPart)]),
+    ("clspmv_plan_partition", C.c_void_p, [C.c_void_p]),
+    ("clspmv_plan_free", C.c_int, [C.c_void_p]),
+    ("clspmv_init_plan_kernels", C.c_int, [C.c_void_p, C.c_void_p, _FP, _FP, _P(C.c_void_p)]),
     ("clspmv_power_iterate", C.c_int, [C.c_void_p, C.c_int]),
     ("clspmv_evaluate", C.c_int, [C.c_void_p, _FP, _FP, _FP]),
     ("clspmv_evaluate_report", C.c_int, [C.c_void_p, C.c_void_p, _FP, _FP, _FP, C.c_int, C.c_int, C.c_void_p]),
@@ -210,6 +226,7 @@ _SIGS = [
     ("clspmv_part_csr_mixed", C.c_int, [C.c_void_p, _P(Csr), C.c_int, _P(C.c_void_p)]),
     ("clspmv_spmm_default_tile", SpmmTile, [C.c_int]),
     ("clspmv_batch_create", C.c_int, [C.c_void_p, C.c_int, _P(_P(Csr)), _P(C.c_int), _P(SpmmTile), _P(C.c_void_p)]),
+    ("clspmv_batch_create_entries", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, _P(SpmmTile), _P(C.c_void_p)]),
     ("clspmv_batch_free", C.c_int, [C.c_void_p]),
     ("clspmv_batch_size", C.c_int, [C.c_void_p]),
     ("clspmv_batch_buffers", C.c_int, [C.c_void_p, C.c_int, _P(C.c_void_p), _P(C.c_void_p)]),

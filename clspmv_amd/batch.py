@@ -1,6 +1,7 @@
 """Batched multi-right-hand-side SpMV (include/clspmv_batch.h).
 
-    b = Batch(ctx, [csr0, csr1, ...], ks=[1, 4, 16, ...])
+    b = Batch(ctx, [csr0, bsr1, ...], ks=[1, 4, 16, ...])   # CSR or BSR r x r
+                                                            # (r = 2..4, k 4 or 16)
     b.set_x(i, X_i)            # X_i: (cols_i, k_i) float32, row-major
     b.capture()                # optional: replay one hipGraph per run
     b.run(1)                   # Y_i += A_i X_i for every i, one launch
@@ -27,15 +28,18 @@ class Batch:
         n = len(self.mats)
         self.ks = [int(k) for k in ks]
         self.shapes = [(m.s.matinfo.height, m.s.matinfo.width) for m in self.mats]
-        ptrs = (C.POINTER(L.Csr) * n)(*[C.pointer(m.s) for m in self.mats])
-        karr = (C.c_int * n)(*self.ks)
+        ents = (L.BatchEntry * n)()
+        for i, m in enumerate(self.mats):
+            is_bsr = isinstance(m.s, L.Bsr)
+            ents[i] = L.BatchEntry(None if is_bsr else C.cast(C.pointer(m.s), C.c_void_p).value,
+                                   C.cast(C.pointer(m.s), C.c_void_p).value if is_bsr else None, self.ks[i])
         tarr = None
         if tiles is not None:
             tarr = (L.SpmmTile * n)()
             for i, t in enumerate(tiles):
                 tarr[i] = t if isinstance(t, L.SpmmTile) else L.SpmmTile(*t)
         self.h = C.c_void_p()
-        check(lib().clspmv_batch_create(ctx.h, n, ptrs, karr, tarr, C.byref(self.h)), "batch_create")
+        check(lib().clspmv_batch_create_entries(ctx.h, n, ents, tarr, C.byref(self.h)), "batch_create_entries")
 
     def __len__(self):
         return len(self.ks)

@@ -96,26 +96,92 @@ def analyze_matrix(coo, tables: BenchTables, dia_priority_multiplier: float = 1.
     return mat
 
 
+FORMAT_NAMES = {L.FMT_CSR_SCALAR: "csr_scalar", L.FMT_CSR_VECTOR: "csr", L.FMT_BDIA: "bdia", L.FMT_DIA: "dia",
+                L.FMT_ELL: "ell", L.FMT_COO: "coo", L.FMT_BELL: "bell", L.FMT_BCSR: "bcsr", L.FMT_SELL: "sell",
+                L.FMT_SBELL: "sbell", L.FMT_BSR: "bsr"}
+
+
+class Plan:
+    """The MI355X plan of a partition (clspmv_analyze_plan /
+    clspmv_plan_cocktail): the parts that run, each with its format, kernel
+    method and row range."""
+
+    def __init__(self, h, mat=None):
+        self.h = h
+        self._mat = mat  # keeps a caller's partition alive
+
+    @classmethod
+    def analyze(cls, coo, tables: BenchTables, dia_priority_multiplier: float = 1.0,
+                estimate_flat_partial: bool = True, estimate_flat_full: bool = False):
+        h = C.c_void_p()
+        check(lib().clspmv_analyze_plan(tables.h, C.byref(coo.s), dia_priority_multiplier,
+                                        int(estimate_flat_partial), int(estimate_flat_full), C.byref(h)),
+              "analyze_plan")
+        return cls(h)
+
+    @classmethod
+    def of(cls, mat: CocktailMatrix):
+        h = C.c_void_p()
+        check(lib().clspmv_plan_cocktail(C.byref(mat.s), C.byref(h)), "plan_cocktail")
+        return cls(h, mat)
+
+    def parts(self):
+        out = []
+        for i in range(lib().clspmv_plan_num_parts(self.h)):
+            q = L.PlanPart()
+            check(lib().clspmv_plan_get_part(self.h, i, C.byref(q)), "plan_get_part")
+            d = {k: getattr(q, k) for k, _ in L.PlanPart._fields_}
+            d["source_name"] = PART_NAMES[q.source] if q.source < 9 else "run"
+            d["format_name"] = FORMAT_NAMES.get(q.format, str(q.format))
+            out.append(d)
+        return out
+
+    def partition(self):
+        """The partition struct the plan runs (a read-only view)."""
+        p = lib().clspmv_plan_partition(self.h)
+        return C.cast(p, C.POINTER(L.Cocktail)).contents
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().clspmv_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class GpuCocktail:
     """cocktail_gpu + cocktail_kernels: the partition on the device."""
 
-    def __init__(self, ctx: Context, mat: CocktailMatrix, vec=None, res=None, use_tex: bool = False):
+    def __init__(self, ctx: Context, mat, vec=None, res=None, use_tex: bool = False):
+        """`mat`: a CocktailMatrix (init_mat_kernels) or a Plan
+        (init_plan_kernels)."""
         self.ctx, self.mat = ctx, mat
         fp = C.POINTER(C.c_float)
         v = None if vec is None else np.ascontiguousarray(vec, np.float32)
         r = None if res is None else np.ascontiguousarray(res, np.float32)
         self.h = C.c_void_p()
-        check(lib().clspmv_init_mat_kernels(ctx.h, C.byref(mat.s), None if v is None else v.ctypes.data_as(fp),
-                                            None if r is None else r.ctypes.data_as(fp), int(use_tex),
-                                            C.byref(self.h)), "init_mat_kernels")
+        vp = None if v is None else v.ctypes.data_as(fp)
+        rp = None if r is None else r.ctypes.data_as(fp)
+        if isinstance(mat, Plan):
+            s = mat.partition()
+            self._shape = (s.mat_width, s.mat_height)
+            check(lib().clspmv_init_plan_kernels(ctx.h, mat.h, vp, rp, C.byref(self.h)), "init_plan_kernels")
+        else:
+            self._shape = (mat.s.mat_width, mat.s.mat_height)
+            check(lib().clspmv_init_mat_kernels(ctx.h, C.byref(mat.s), vp, rp, int(use_tex), C.byref(self.h)),
+                  "init_mat_kernels")
 
     @property
     def width(self):
-        return self.mat.s.mat_width
+        return self._shape[0]
 
     @property
     def height(self):
-        return self.mat.s.mat_height
+        return self._shape[1]
 
     def do_spmv(self, ntimes: int = 1):
         check(lib().clspmv_do_spmv(self.h, ntimes), "do_spmv")

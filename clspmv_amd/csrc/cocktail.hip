@@ -162,40 +162,40 @@ bool xtile_pays(long long rows, long long cols, long long nnz) {
 //       tuned workgroup count and balances any row-length distribution.
 //  COO  segmented reduction.
 //  Wide x: see xtile_pays above.
+// Part `which` (0..8: bdia, dia, sbell, bell, bcsr, sell, ell, csr, coo) of
+// the partition as the format it is uploaded as and the method passed to
+// its clspmv_part_* call.
+void partition_part(int which, const clspmv_cocktail* m, int* fmt, int* meth) {
+    auto flat = [&](const clspmv_matinfo& a, int near_meth) {
+        return !wide_x(a.width) ? near_meth : xtile_pays(a.height, a.width, a.nnz) ? kXtile : 2;
+    };
+    switch (which) {
+        case 0: *fmt = CLSPMV_FMT_BDIA, *meth = m->bdia_meth_num; break;
+        case 1: *fmt = CLSPMV_FMT_DIA, *meth = m->dia_meth_num; break;
+        case 2: *fmt = CLSPMV_FMT_SBELL, *meth = m->sbell_meth_num; break;
+        case 3: *fmt = CLSPMV_FMT_BELL, *meth = m->bell_meth_num; break;
+        case 4: *fmt = CLSPMV_FMT_BCSR, *meth = m->bcsr_meth_num; break;
+        case 5: *fmt = CLSPMV_FMT_SELL, *meth = flat(m->sell.matinfo, m->sell_meth_num); break;
+        case 6: *fmt = CLSPMV_FMT_ELL, *meth = flat(m->ell.matinfo, m->ell_meth_num == 1 ? 1 : 0); break;
+        case 7: *fmt = CLSPMV_FMT_CSR_VECTOR, *meth = flat(m->csr.matinfo, 1); break;
+        default: *fmt = CLSPMV_FMT_COO, *meth = flat(m->coo.matinfo, 0); break;
+    }
+}
+
 int add_part(clspmv_gpumat_s* g, int which, const clspmv_cocktail* m) {
     clspmv_part_t p = nullptr;
-    int rc = CLSPMV_OK;
+    int fmt = 0, meth = 0, rc = CLSPMV_OK;
+    partition_part(which, m, &fmt, &meth);
     switch (which) {
-        case 0: rc = clspmv_part_bdia(g->ctx, &m->bdia, m->bdia_meth_num, &p); break;
-        case 1: rc = clspmv_part_dia(g->ctx, &m->dia, m->dia_meth_num, &p); break;
-        case 2: rc = clspmv_part_sbell(g->ctx, &m->sbell, m->sbell_meth_num, &p); break;
-        case 3: rc = clspmv_part_bell(g->ctx, &m->bell, m->bell_meth_num, &p); break;
-        case 4: rc = clspmv_part_bcsr(g->ctx, &m->bcsr, m->bcsr_meth_num, &p); break;
-        case 5: {
-            const auto& a = m->sell.matinfo;
-            const int meth = !wide_x(a.width) ? m->sell_meth_num : xtile_pays(a.height, a.width, a.nnz) ? kXtile : 2;
-            rc = clspmv_part_sell(g->ctx, &m->sell, meth, &p);
-            break;
-        }
-        case 6: {
-            const auto& a = m->ell.matinfo;
-            const int meth = !wide_x(a.width) ? (m->ell_meth_num == 1 ? 1 : 0)
-                             : xtile_pays(a.height, a.width, a.nnz) ? kXtile : 2;
-            rc = clspmv_part_ell(g->ctx, &m->ell, meth, &p);
-            break;
-        }
-        case 7: {
-            const auto& a = m->csr.matinfo;
-            const int meth = !wide_x(a.width) ? 1 : xtile_pays(a.height, a.width, a.nnz) ? kXtile : 2;
-            rc = clspmv_part_csr_vector(g->ctx, &m->csr, meth, &p);
-            break;
-        }
-        case 8: {
-            const auto& a = m->coo.matinfo;
-            const int meth = !wide_x(a.width) ? 0 : xtile_pays(a.height, a.width, a.nnz) ? kXtile : 2;
-            rc = clspmv_part_coo(g->ctx, &m->coo, meth, &p);
-            break;
-        }
+        case 0: rc = clspmv_part_bdia(g->ctx, &m->bdia, meth, &p); break;
+        case 1: rc = clspmv_part_dia(g->ctx, &m->dia, meth, &p); break;
+        case 2: rc = clspmv_part_sbell(g->ctx, &m->sbell, meth, &p); break;
+        case 3: rc = clspmv_part_bell(g->ctx, &m->bell, meth, &p); break;
+        case 4: rc = clspmv_part_bcsr(g->ctx, &m->bcsr, meth, &p); break;
+        case 5: rc = clspmv_part_sell(g->ctx, &m->sell, meth, &p); break;
+        case 6: rc = clspmv_part_ell(g->ctx, &m->ell, meth, &p); break;
+        case 7: rc = clspmv_part_csr_vector(g->ctx, &m->csr, meth, &p); break;
+        case 8: rc = clspmv_part_coo(g->ctx, &m->coo, meth, &p); break;
         default: return CLSPMV_E_INVALID;
     }
     if (rc) return rc;
@@ -316,12 +316,21 @@ signed char classify_block(const clspmv_csr_matrix& m, const std::vector<int>& r
     return 4 * slots <= 5 * n ? kLocalSell : kLocalCsr;
 }
 
-// One run as a part of its class; rows [0, sub.height) of `sub`.
-int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char cls, int row0) {
+// Method of a row run of class cls: the layout its part is built with.
+int run_method(signed char cls, int rows, int cols, long long nnz) {
+    if (cls == kScattered) return !wide_x(cols) ? 1 : xtile_pays(rows, cols, nnz) ? kXtile : 2;
+    return cls == kLocalCsr ? 1 : 0;
+}
+int run_format(signed char cls) {
+    return cls == kLocalSell ? CLSPMV_FMT_SELL : (cls == kLocalBsr2 || cls == kLocalBsr4) ? CLSPMV_FMT_BSR
+                                                                                          : CLSPMV_FMT_CSR_VECTOR;
+}
+
+// One run as a part of its class and method; rows [0, sub.height) of `sub`.
+int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char cls, int meth, int row0) {
     const int cols = sub.matinfo.width, rows = sub.matinfo.height;
     const long long n = sub.matinfo.nnz;
-    if (cls == kScattered) {
-        const int meth = !wide_x(cols) ? 1 : xtile_pays(rows, cols, n) ? kXtile : 2;
+    if (cls == kScattered || cls == kLocalCsr) {
         if (meth != kXtile || !g_run_codec) return add_flat_part(g, sub, meth, row0);
         clspmv_part_t p = nullptr;
         const int rc = clspmv_part_csr_mixed(g->ctx, &sub, g_run_codec, &p);
@@ -331,7 +340,6 @@ int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char c
         g->row0.push_back(row0);
         return CLSPMV_OK;
     }
-    if (cls == kLocalCsr) return add_flat_part(g, sub, 1, row0);
     // SELL / BSR: through the host conversions of a COO copy of the run
     clspmv_coo_matrix coo{};
     int rc = clspmv_alloc_coo(&coo, cols, rows, (int)n);
@@ -366,13 +374,57 @@ int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char c
     return CLSPMV_OK;
 }
 
-int add_flat_rows(clspmv_gpumat_s* g, const clspmv_csr_matrix& merged, const std::vector<int>& rp, bool single_ok) {
-    const int rows = merged.matinfo.height, cols = merged.matinfo.width;
+}  // namespace
+
+// The plan: which parts a Cocktail partition runs as on this GPU.  Parts
+// 0..8 of the partition (bdia, dia, sbell, bell, bcsr, sell, ell, csr, coo)
+// keep their format; the flat ones (5..8) are usually replaced by row runs
+// of their merged rows (source 9), each with the class and method chosen
+// here.  Built on the host (analysis only), uploaded by
+// clspmv_init_plan_kernels.
+struct PlanPart {
+    int source;        // 0..8 partition part, 9 row run
+    signed char cls;   // row runs: RunClass
+    int format, method;
+    int row0, row1;
+    long long nnz;
+};
+struct clspmv_plan_s {
+    clspmv_cocktail own{};  // the partition, when the plan owns it (clspmv_analyze_plan)
+    bool owns = false;
+    const clspmv_cocktail* m = nullptr;
+    std::vector<int> rp, ci;  // merged flat rows (CSR), when merged
+    std::vector<float> cv;
+    std::vector<PlanPart> parts;
+    ~clspmv_plan_s() {
+        if (owns) clspmv_free_cocktail(&own);
+    }
+};
+
+namespace {
+
+// Row runs of the merged flat rows (rp, ci, cv of the plan).  Returns
+// kOneRun (nothing planned) when single_ok is false and the rows form a
+// single run of a class that keeps the partition's own flat parts.
+int plan_runs(clspmv_plan_s* P, bool single_ok) {
+    const int rows = P->m->mat_height, cols = P->m->mat_width;
+    clspmv_csr_matrix merged{};
+    merged.matinfo.width = cols;
+    merged.matinfo.height = rows;
+    merged.matinfo.nnz = rows ? P->rp[(size_t)rows] : 0;
+    merged.csr_row_ptr = P->rp.data();
+    merged.csr_col_id = P->ci.data();
+    merged.csr_data = P->cv.data();
+    const std::vector<int>& rp = P->rp;
+    auto push_run = [&](signed char cls, int r0, int r1) {
+        const long long n = rp[(size_t)r1] - rp[(size_t)r0];
+        P->parts.push_back(PlanPart{9, cls, run_format(cls), run_method(cls, r1 - r0, cols, n), r0, r1, n});
+    };
     const int nbk = (rows + kRunBlock - 1) / kRunBlock;
     if (!g_split_local || nbk == 0) {
         if (!single_ok) return kOneRun;
-        const long long n = rows ? rp[(size_t)rows] : 0;
-        return add_flat_part(g, merged, !wide_x(cols) ? 1 : xtile_pays(rows, cols, n) ? kXtile : 2, 0);
+        push_run(kScattered, 0, rows);  // one CSR part, by the cost rule (local x: adaptive rows)
+        return CLSPMV_OK;
     }
     std::vector<signed char> cls((size_t)nbk);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(omp_threads())
@@ -380,40 +432,26 @@ int add_flat_rows(clspmv_gpumat_s* g, const clspmv_csr_matrix& merged, const std
         cls[(size_t)b] = classify_block(merged, rp, b * kRunBlock, std::min(rows, (b + 1) * kRunBlock));
     for (int b = 0; b < nbk; ++b)  // empty blocks join the previous run
         if (cls[(size_t)b] < 0) cls[(size_t)b] = b ? cls[(size_t)b - 1] : kLocalCsr;
-    // runs of equal class
     std::vector<int> cut{0};
     for (int b = 1; b < nbk; ++b)
         if (cls[(size_t)b] != cls[(size_t)b - 1]) cut.push_back(b * kRunBlock);
     cut.push_back(rows);
-    if (cut.size() == 2) {
+    if (cut.size() == 2 && cls[0] < kLocalBsr2) {
         // one run: only a block-structured one changes layout (to sliced
-        // BSR); otherwise the partition's own flat part (single_ok false)
-        // or the merged rows as before
-        if (cls[0] < kLocalBsr2) {
-            if (!single_ok) return kOneRun;
-            const long long n = rp[(size_t)rows];
-            return add_flat_part(g, merged, !wide_x(cols) ? 1 : xtile_pays(rows, cols, n) ? kXtile : 2, 0);
-        }
+        // BSR); otherwise the partition's own flat parts (single_ok false)
+        // or the merged rows as one CSR part
+        if (!single_ok) return kOneRun;
+        push_run(kScattered, 0, rows);
+        return CLSPMV_OK;
     }
-    for (size_t i = 0; i + 1 < cut.size(); ++i) {
-        const int r0 = cut[i], r1 = cut[i + 1];
-        std::vector<int> sub_rp((size_t)(r1 - r0) + 1);
-        for (int r = r0; r <= r1; ++r) sub_rp[(size_t)(r - r0)] = rp[(size_t)r] - rp[(size_t)r0];
-        clspmv_csr_matrix sub{};
-        sub.matinfo.width = cols;
-        sub.matinfo.height = r1 - r0;
-        sub.matinfo.nnz = sub_rp.back();
-        sub.csr_row_ptr = sub_rp.data();
-        sub.csr_col_id = merged.csr_col_id + rp[(size_t)r0];
-        sub.csr_data = merged.csr_data + rp[(size_t)r0];
-        const int rc = add_run_part(g, sub, cls[(size_t)(r0 / kRunBlock)], r0);
-        if (rc) return rc;
-    }
+    for (size_t i = 0; i + 1 < cut.size(); ++i) push_run(cls[(size_t)(cut[i] / kRunBlock)], cut[i], cut[i + 1]);
     return CLSPMV_OK;
 }
 
-int merge_flat(clspmv_gpumat_s* g, const clspmv_cocktail* m, bool single_ok) {
-    const int rows = m->mat_height, cols = m->mat_width;
+// Merges the flat parts' rows into P->rp / ci / cv.
+int merge_flat(clspmv_plan_s* P) {
+    const clspmv_cocktail* m = P->m;
+    const int rows = m->mat_height;
     std::vector<long long> cnt((size_t)rows + 1, 0);
     const clspmv_ell_matrix& e = m->ell;
     const clspmv_sell_matrix& sl = m->sell;
@@ -426,8 +464,8 @@ int merge_flat(clspmv_gpumat_s* g, const clspmv_cocktail* m, bool single_ok) {
         for (int r = 0; r < rows; ++r) coo_rp[(size_t)r + 1] += coo_rp[(size_t)r];
     }
     auto sell_width = [&](int r) {
-        const int S = (int)sl.sell_slice_height, s = r / S;
-        return (sl.sell_slice_ptr[s + 1] - sl.sell_slice_ptr[s]) / S;
+        const int S = (int)sl.sell_slice_height, sidx = r / S;
+        return (sl.sell_slice_ptr[sidx + 1] - sl.sell_slice_ptr[sidx]) / S;
     };
     auto ell_slot = [&](int r, int k) { return (size_t)k * e.ell_height_aligned + (size_t)r; };
     auto sell_slot = [&](int r, int j) {
@@ -447,9 +485,13 @@ int merge_flat(clspmv_gpumat_s* g, const clspmv_cocktail* m, bool single_ok) {
     }
     for (int r = 0; r < rows; ++r) cnt[(size_t)r + 1] += cnt[(size_t)r];
     if (cnt[(size_t)rows] > INT32_MAX) return CLSPMV_E_TOO_LARGE;
-    std::vector<int> rp((size_t)rows + 1), ci((size_t)cnt[(size_t)rows]);
-    std::vector<float> cv((size_t)cnt[(size_t)rows]);
+    std::vector<int>& rp = P->rp;
+    rp.assign((size_t)rows + 1, 0);
+    P->ci.assign((size_t)cnt[(size_t)rows], 0);
+    P->cv.assign((size_t)cnt[(size_t)rows], 0.f);
     for (int r = 0; r <= rows; ++r) rp[(size_t)r] = (int)cnt[(size_t)r];
+    int* ci = P->ci.data();
+    float* cv = P->cv.data();
 #pragma omp parallel for schedule(dynamic, 1024) num_threads(omp_threads())
     for (int r = 0; r < rows; ++r) {
         std::vector<std::pair<int, float>> row;
@@ -474,15 +516,63 @@ int merge_flat(clspmv_gpumat_s* g, const clspmv_cocktail* m, bool single_ok) {
             cv[(size_t)rp[(size_t)r] + k] = row[k].second;
         }
     }
-    clspmv_csr_matrix merged{};
-    merged.matinfo.width = cols;
-    merged.matinfo.height = rows;
-    merged.matinfo.nnz = (m->ifuseell ? e.matinfo.nnz : 0) + (m->ifusesell ? sl.matinfo.nnz : 0) +
-                         (m->ifusecsr ? c.matinfo.nnz : 0) + (m->ifusecoo ? o.matinfo.nnz : 0);
-    merged.csr_row_ptr = rp.data();
-    merged.csr_col_id = ci.data();
-    merged.csr_data = cv.data();
-    return add_flat_rows(g, merged, rp, single_ok);
+    return CLSPMV_OK;
+}
+
+// The plan of partition m (not owned: it must outlive the plan's upload).
+int build_plan(const clspmv_cocktail* m, clspmv_plan_s* P) {
+    P->m = m;
+    P->parts.clear();
+    const unsigned char use[9] = {m->ifusebdia, m->ifusedia, m->ifusesbell, m->ifusebell, m->ifusebcsr,
+                                  m->ifusesell, m->ifuseell, m->ifusecsr, m->ifusecoo};
+    const int nflat = use[5] + use[6] + use[7] + use[8];
+    // a single flat part goes through the merge too when it may split into
+    // row runs; if it does not, it keeps its own format
+    const bool merge = g_merge_flat && (nflat >= 2 || (nflat == 1 && g_split_local));
+    const long long nnz_of[9] = {m->bdia.matinfo.nnz, m->dia.matinfo.nnz, m->sbell.matinfo.nnz,
+                                 m->bell.matinfo.nnz, m->bcsr.matinfo.nnz, m->sell.matinfo.nnz,
+                                 m->ell.matinfo.nnz, m->csr.matinfo.nnz, m->coo.matinfo.nnz};
+    auto push_part = [&](int i) {
+        int fmt = 0, meth = 0;
+        partition_part(i, m, &fmt, &meth);
+        P->parts.push_back(PlanPart{i, -1, fmt, meth, 0, m->mat_height, nnz_of[i]});
+    };
+    for (int i = 0; i < 9; ++i)
+        if (use[i] && !(merge && i >= 5)) push_part(i);
+    if (!merge) return CLSPMV_OK;
+    int rc = merge_flat(P);
+    if (!rc) rc = plan_runs(P, nflat >= 2);
+    if (rc == kOneRun) {
+        P->rp.clear();
+        P->ci.clear();
+        P->cv.clear();
+        for (int i = 5; i < 9; ++i)
+            if (use[i]) push_part(i);
+        rc = CLSPMV_OK;
+    }
+    return rc;
+}
+
+int upload_plan(clspmv_gpumat_s* g, const clspmv_plan_s* P) {
+    for (const PlanPart& q : P->parts) {
+        int rc;
+        if (q.source < 9) {
+            rc = add_part(g, q.source, P->m);
+        } else {
+            std::vector<int> sub_rp((size_t)(q.row1 - q.row0) + 1);
+            for (int r = q.row0; r <= q.row1; ++r) sub_rp[(size_t)(r - q.row0)] = P->rp[(size_t)r] - P->rp[(size_t)q.row0];
+            clspmv_csr_matrix sub{};
+            sub.matinfo.width = P->m->mat_width;
+            sub.matinfo.height = q.row1 - q.row0;
+            sub.matinfo.nnz = sub_rp.back();
+            sub.csr_row_ptr = sub_rp.data();
+            sub.csr_col_id = const_cast<int*>(P->ci.data()) + P->rp[(size_t)q.row0];
+            sub.csr_data = const_cast<float*>(P->cv.data()) + P->rp[(size_t)q.row0];
+            rc = add_run_part(g, sub, q.cls, q.method, q.row0);
+        }
+        if (rc) return rc;
+    }
+    return CLSPMV_OK;
 }
 
 int check_size(clspmv_gpumat_t g, int n, int limit) {
@@ -493,14 +583,8 @@ int check_size(clspmv_gpumat_t g, int n, int limit) {
     return CLSPMV_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* m, const float* vec, const float* res,
-                            int use_tex, clspmv_gpumat_t* out) {
-    (void)use_tex;
-    if (!ctx || !m || !out) return CLSPMV_E_INVALID;
+int init_from_plan(clspmv_ctx_t ctx, const clspmv_plan_s* P, const float* vec, const float* res, clspmv_gpumat_t* out) {
+    const clspmv_cocktail* m = P->m;
     auto* g = new (std::nothrow) clspmv_gpumat_s();
     if (!g) return CLSPMV_E_NOMEM;
     g->ctx = ctx;
@@ -514,22 +598,7 @@ int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* m, const fl
     if (!rc) rc = hip_status(hipMemsetAsync(g->y.ptr, 0, yn * 4, ctx->stream()), "hipMemset");
     if (!rc && vec) rc = clspmv_memcpy_h2d(ctx, g->x.ptr, vec, (size_t)g->width * 4);
     if (!rc && res) rc = clspmv_memcpy_h2d(ctx, g->y.ptr, res, (size_t)g->height * 4);
-    const unsigned char use[9] = {m->ifusebdia, m->ifusedia, m->ifusesbell, m->ifusebell, m->ifusebcsr,
-                                  m->ifusesell, m->ifuseell, m->ifusecsr, m->ifusecoo};
-    const int nflat = use[5] + use[6] + use[7] + use[8];
-    // a single flat part goes through the merge too when it may split into
-    // row runs; if it does not, it keeps its own format
-    const bool merge = g_merge_flat && (nflat >= 2 || (nflat == 1 && g_split_local));
-    for (int i = 0; i < 9 && !rc; ++i)
-        if (use[i] && !(merge && i >= 5)) rc = add_part(g, i, m);
-    if (!rc && merge) {
-        rc = merge_flat(g, m, nflat >= 2);
-        if (rc == kOneRun) {
-            rc = CLSPMV_OK;
-            for (int i = 5; i < 9 && !rc; ++i)
-                if (use[i]) rc = add_part(g, i, m);
-        }
-    }
+    if (!rc) rc = upload_plan(g, P);
     if (!rc) rc = clspmv_synchronize(ctx);
     if (rc) {
         delete g;
@@ -538,6 +607,74 @@ int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* m, const fl
     }
     *out = g;
     return CLSPMV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* m, const float* vec, const float* res,
+                            int use_tex, clspmv_gpumat_t* out) {
+    (void)use_tex;
+    if (!ctx || !m || !out) return CLSPMV_E_INVALID;
+    clspmv_plan_s plan;
+    const int rc = build_plan(m, &plan);
+    if (rc) return rc;
+    return init_from_plan(ctx, &plan, vec, res, out);
+}
+
+int clspmv_plan_cocktail(const clspmv_cocktail* m, clspmv_plan_t* out) {
+    if (!m || !out) return CLSPMV_E_INVALID;
+    auto* P = new (std::nothrow) clspmv_plan_s();
+    if (!P) return CLSPMV_E_NOMEM;
+    const int rc = build_plan(m, P);
+    if (rc) {
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return CLSPMV_OK;
+}
+
+int clspmv_analyze_plan(clspmv_bench_t bench, clspmv_coo_matrix* coo, float dia_priority_multiplier,
+                        int estimate_flat_partial, int estimate_flat_full, clspmv_plan_t* out) {
+    if (!bench || !coo || !out) return CLSPMV_E_INVALID;
+    auto* P = new (std::nothrow) clspmv_plan_s();
+    if (!P) return CLSPMV_E_NOMEM;
+    clspmv_init_cocktail(&P->own);
+    P->owns = true;
+    int rc = clspmv_analyze_matrix(bench, coo, &P->own, dia_priority_multiplier, estimate_flat_partial,
+                                   estimate_flat_full, 0);
+    if (!rc) rc = build_plan(&P->own, P);
+    if (rc) {
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return CLSPMV_OK;
+}
+
+int clspmv_plan_num_parts(clspmv_plan_t P) { return P ? (int)P->parts.size() : 0; }
+
+int clspmv_plan_get_part(clspmv_plan_t P, int i, clspmv_plan_part* out) {
+    if (!P || !out || i < 0 || i >= (int)P->parts.size()) return CLSPMV_E_INVALID;
+    const PlanPart& q = P->parts[(size_t)i];
+    *out = clspmv_plan_part{q.source, q.format, q.method, q.row0, q.row1, q.nnz};
+    return CLSPMV_OK;
+}
+
+const clspmv_cocktail* clspmv_plan_partition(clspmv_plan_t P) { return P ? P->m : nullptr; }
+
+int clspmv_plan_free(clspmv_plan_t P) {
+    if (!P) return CLSPMV_E_INVALID;
+    delete P;
+    return CLSPMV_OK;
+}
+
+int clspmv_init_plan_kernels(clspmv_ctx_t ctx, clspmv_plan_t P, const float* vec, const float* res,
+                             clspmv_gpumat_t* out) {
+    if (!ctx || !P || !out) return CLSPMV_E_INVALID;
+    return init_from_plan(ctx, P, vec, res, out);
 }
 
 int clspmv_free_mat_kernels(clspmv_gpumat_t g) {

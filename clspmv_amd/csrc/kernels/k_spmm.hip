@@ -18,6 +18,13 @@
 //     every slot strides the chunk, the slots' partials are summed with
 //     shuffles and written to carry slot `slot` (16 floats); spmm_finalize
 //     adds the carries of each long row in chunk order.
+//   BLOCK {mat, b0, b1, 0} of a BSR matrix (dense bdim x bdim blocks,
+//     bdim = 2..4, k = 4 or 16): block rows [b0, b1), at most 16 / (k/4),
+//     on the matrix cores.  One v_mfma_f32_4x4x1f32 is 16 independent 4x4
+//     outer products; here each is (block row g, right-hand-side group q):
+//     rows of the block row's current block, column c, times x[col + c]
+//     [4q .. 4q+3], so bdim instructions advance every block row of the
+//     item by one block (see spmm_block).
 // Waves take items i, i + W, i + 2W, ... (W = waves in the grid), so the
 // launch is one grid whatever the number of matrices.  No atomics: results
 // are deterministic.
@@ -145,6 +152,57 @@ __device__ void spmm_long(const SpmmMat& m, int row, int slot_id, int chunk, flo
     if (s == 0) acc.store(carry + (size_t)slot_id * 16 + v * Ln::W);
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// One BLOCK item: block rows [b0, b1) of a BSR matrix with B x B blocks and
+// K in {4, 16} right-hand sides.  v_mfma_f32_4x4x1f32 works on 16 blocks of
+// four lanes: lane 4 * blk + t gives row t of the block's 4 x 1 operand A
+// and column t of its 1 x 4 operand B, and holds column t of the 4 x 4
+// result (rows in its four accumulator registers).  Block blk is (block row
+// g = blk / Q, right-hand-side group q = blk % Q), Q = K / 4; so with
+// A = column c of g's current block (rows >= B read 0) and B = x[col + c]
+// [4q .. 4q+3], the c = 0 .. B-1 steps add that block's product to the
+// block row's 4 x 4 tile of Y.  Every block row of the item advances by one
+// block per B steps; rows with fewer blocks feed zeros (and a valid x).
+template <int K, int B>
+__device__ void spmm_block(const SpmmMat& m, int b0, int b1, int lane) {
+    constexpr int Q = K / 4, G = 16 / Q;
+    const int blk = lane >> 2, t = lane & 3;
+    const int g = blk / Q, q = blk % Q;
+    const int br = b0 + g;
+    const bool row_ok = br < b1;
+    const int s = row_ok ? m.row_ptr[br] : 0, e = row_ok ? m.row_ptr[br + 1] : 0;
+    // blocks of the longest block row of the item (wave-uniform)
+    int n = e - s;
+#pragma unroll
+    for (int d = 4; d < 64; d <<= 1) n = max(n, __shfl_xor(n, d, 64));
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const float* xq = m.x + q * 4 + t;
+    for (int j = 0; j < n; ++j) {
+        const bool on = j < e - s;
+        const int kb = on ? s + j : 0;
+        const int c0 = on ? m.col[kb] : 0;
+        const float* a = m.val + (size_t)kb * (B * B) + t * B;
+        float av[B], xv[B];
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+            av[c] = (on && t < B) ? a[c] : 0.f;
+            const int col = min(c0 + c, m.cols - 1);  // block columns past the last read zeros in A
+            xv[c] = xq[(size_t)col * K];
+        }
+#pragma unroll
+        for (int c = 0; c < B; ++c) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(av[c], xv[c], acc, 0, 0, 0);
+    }
+    if (row_ok) {
+#pragma unroll
+        for (int v = 0; v < B; ++v) {
+            const int r = br * B + v;
+            if (r < m.rows) m.y[(size_t)r * K + q * 4 + t] += acc[v];
+        }
+    }
+    (void)G;
+}
+
 __global__ __launch_bounds__(256) void spmm_kernel(const SpmmMat* __restrict__ mats, const int4* __restrict__ items,
                                                    int n_items, float* __restrict__ carry) {
     __shared__ int s_col[4][kSpmmStageNnz];
@@ -154,7 +212,17 @@ __global__ __launch_bounds__(256) void spmm_kernel(const SpmmMat* __restrict__ m
     for (int it = blockIdx.x * 4 + w; it < n_items; it += waves) {
         const int4 d = items[it];
         const SpmmMat m = mats[d.x];
-        if (d.z >= 0) {
+        if (m.bdim) {
+            // BSR items (host-built: k = 4 or 16, bdim = 2..4)
+            switch (m.k * 10 + m.bdim) {
+                case 42: spmm_block<4, 2>(m, d.y, d.z, lane); break;
+                case 43: spmm_block<4, 3>(m, d.y, d.z, lane); break;
+                case 44: spmm_block<4, 4>(m, d.y, d.z, lane); break;
+                case 162: spmm_block<16, 2>(m, d.y, d.z, lane); break;
+                case 163: spmm_block<16, 3>(m, d.y, d.z, lane); break;
+                default: spmm_block<16, 4>(m, d.y, d.z, lane); break;
+            }
+        } else if (d.z >= 0) {
             switch (m.k) {
                 case 1: spmm_stream<1>(m, d.y, d.z, s_col[w], s_val[w], lane); break;
                 case 4: spmm_stream<4>(m, d.y, d.z, s_col[w], s_val[w], lane); break;

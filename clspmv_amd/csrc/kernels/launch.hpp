@@ -167,16 +167,19 @@ struct DevBsrMixed {
 constexpr int kSpmmStageNnz = 512;    // entries of a STREAM item (staged in LDS)
 constexpr int kSpmmLongChunk = 1024;  // entries of a LONG item
 struct SpmmMat {
-    const int* row_ptr;
-    const int* col;
-    const float* val;
-    const float* x;  // cols x k, row-major
-    float* y;        // rows x k, row-major
-    int rows, k;     // k in {1, 4, 16}
+    const int* row_ptr;  // CSR: rows + 1; BSR: block rows + 1
+    const int* col;      // CSR: column; BSR: first column of each block
+    const float* val;    // CSR: value; BSR: bdim x bdim per block, row-major
+    const float* x;      // cols x k, row-major
+    float* y;            // rows x k, row-major
+    int rows, k;         // k in {1, 4, 16}
+    int bdim;            // 0: CSR; 2..4: BSR with bdim x bdim blocks (k = 4 or 16)
+    int cols;
 };
 struct DevSpmm {
     const SpmmMat* mats;
-    const int4* items;   // STREAM {mat, r0, r1, 0} / LONG {mat, row, -(slot+1), chunk}
+    const int4* items;   // STREAM {mat, r0, r1, 0} / LONG {mat, row, -(slot+1), chunk} /
+                         // BLOCK {mat, block row 0, block row 1, 0} of a BSR matrix
     const int4* longs;   // {mat, row, first carry slot, chunks}
     float* carry;        // 16 floats per LONG item
     int n_items, n_long, grid;

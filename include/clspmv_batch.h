@@ -39,6 +39,20 @@ clspmv_spmm_tile clspmv_spmm_default_tile(int k);
  * NULL (defaults).  X_i and Y_i start at zero. */
 int clspmv_batch_create(clspmv_ctx_t ctx, int n, const clspmv_csr_matrix* const* mats, const int* ks,
                         const clspmv_spmm_tile* tiles, clspmv_batch_t* out);
+/* One matrix of a batch: a CSR matrix (bsr NULL) or a BSR matrix with
+ * dense r x r blocks, r = 2..4 (csr NULL; k must be 4 or 16), whose blocks
+ * run on the matrix cores (v_mfma_f32_4x4x1f32: each block times k
+ * right-hand sides is a dense contraction).  The caller chooses per matrix,
+ * e.g. from the format selector. */
+typedef struct {
+    const clspmv_csr_matrix* csr;
+    const clspmv_bsr_matrix* bsr;
+    int k;
+} clspmv_batch_entry;
+/* As clspmv_batch_create, for a mix of CSR and BSR matrices (tiles apply
+ * to the CSR ones and may be NULL). */
+int clspmv_batch_create_entries(clspmv_ctx_t ctx, int n, const clspmv_batch_entry* entries,
+                                const clspmv_spmm_tile* tiles, clspmv_batch_t* out);
 int clspmv_batch_free(clspmv_batch_t b);
 int clspmv_batch_size(clspmv_batch_t b);
 int clspmv_batch_buffers(clspmv_batch_t b, int i, void** d_x, void** d_y);

@@ -128,6 +128,41 @@ int clspmv_cpy_result_to_gpu(clspmv_gpumat_t gpumat, void* d_res, int res_size);
 /* Device pointers of the object's x and y (the reference reads them as
  * gpumat.vec / gpumat.res, e.g. A/src/eval.cpp:50-51). */
 int clspmv_gpumat_buffers(clspmv_gpumat_t gpumat, void** d_vec, void** d_res);
+/* ------------------------------------------------------------------ */
+/* MI355X mode of the analyzer: the plan.  A Cocktail partition keeps the
+ * reference's nine parts; on MI355X the flat ones (ELL, SELL, CSR, COO) are
+ * merged into one set of rows and cut into runs of 32768-row blocks by where
+ * their columns lie and their shape -- rows of dense aligned 4x4 (2x2)
+ * blocks as sliced BSR, near-equal rows as SELL, other local rows as CSR
+ * row blocks, scattered rows in the LDS x-tiled or column-sliced layout by
+ * a fitted cost rule.  The plan lists the parts that will run, each with
+ * its format, kernel method and row range; clspmv_init_mat_kernels
+ * executes the plan of the partition it is given.  Extends analyze_matrix
+ * (A/src/analyze.cpp:3714-3900), whose partition stays bit-exact. */
+typedef struct clspmv_plan_s* clspmv_plan_t;
+typedef struct {
+    int source;       /* 0..8: part of the partition (bdia, dia, sbell, bell, bcsr, sell, ell, csr, coo);
+                         9: a row run of its merged flat rows */
+    int format;       /* CLSPMV_FMT_* the part runs as */
+    int method;       /* the method its clspmv_part_* call receives (4: LDS x-tiled, 2: column-sliced) */
+    int row0, row1;   /* rows it covers */
+    long long nnz;
+} clspmv_plan_part;
+/* analyze_matrix (these tables and knobs) and the plan of its partition;
+ * the plan owns the partition. */
+int clspmv_analyze_plan(clspmv_bench_t bench, clspmv_coo_matrix* coo, float dia_priority_multiplier,
+                        int estimate_flat_partial, int estimate_flat_full, clspmv_plan_t* plan);
+/* The plan of a caller's partition (not copied: it must outlive the plan). */
+int clspmv_plan_cocktail(const clspmv_cocktail* mat, clspmv_plan_t* plan);
+int clspmv_plan_num_parts(clspmv_plan_t plan);
+int clspmv_plan_get_part(clspmv_plan_t plan, int i, clspmv_plan_part* part);
+/* The partition the plan runs (owned by the plan after clspmv_analyze_plan). */
+const clspmv_cocktail* clspmv_plan_partition(clspmv_plan_t plan);
+int clspmv_plan_free(clspmv_plan_t plan);
+/* init_mat_kernels for a plan: uploads its parts, x and y. */
+int clspmv_init_plan_kernels(clspmv_ctx_t ctx, clspmv_plan_t plan, const float* vec, const float* res,
+                             clspmv_gpumat_t* gpumat);
+
 /* Per-part description: count and, for part i, its format/method/kernel,
  * nnz and stored bytes (same fields as clspmv_part_info). */
 int clspmv_gpumat_num_parts(clspmv_gpumat_t gpumat);

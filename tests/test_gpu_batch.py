@@ -84,3 +84,57 @@ def test_batch_graph_replays_match_eager(ctx):
         for i in range(len(coos)):
             assert np.array_equal(b.get_y(i).view(np.uint32), eager[i].view(np.uint32)), (rep, i)
     b.free()
+
+
+def _block_cases():
+    # (coo, block dim): dense blocks, ragged edges (rows and columns not a
+    # multiple of the block), uneven blocks per block row and empty block
+    # rows, and a matrix whose entries do not fill their blocks
+    blocks3 = M.gen_block_rows(700, 3, 9, 40, 11)
+    blocks4 = M.gen_block_rows(300, 4, 6, 30, 12)
+    blocks2 = M.gen_block_rows(900, 2, 7, 60, 13)
+    keep = (blocks3.row_id < 600) | (blocks3.row_id >= 900)
+    holes = M.CooMatrix.from_arrays(blocks3.row_id[keep].copy(), blocks3.col_id[keep].copy(),
+                                    blocks3.data[keep].copy(), blocks3.width, blocks3.height)
+    return [(blocks3, 3), (blocks4, 4), (blocks2, 2), (holes, 3), (rand_coo(301, 299, 3000, 14), 3),
+            (rand_coo(37, 41, 300, 15), 4)]
+
+
+@pytest.mark.parametrize("k", [4, 16])
+def test_batch_bsr_blocks_on_matrix_cores(ctx, k):
+    # BSR entries of a batch (block items on v_mfma_f32_4x4x1f32) beside
+    # CSR entries, every right-hand side against the oracle; graph replays
+    # bitwise equal to the eager run
+    cases = _block_cases()
+    coos = [c for c, _ in cases] + [M.gen_zipf_rows(5000, 1.8, 16.0, 3000, 16)]
+    mats = [M.coo2bsr(c, b, b) for c, b in cases] + [M.coo2csr(coos[-1])]
+    ks = [k] * len(mats)
+    b = B.Batch(ctx, mats, ks)
+    rng = np.random.default_rng(k)
+    xs, y0s = [], []
+    for i, c in enumerate(coos):
+        xs.append(rng.uniform(-1, 1, (c.width, k)).astype(np.float32))
+        y0s.append(rng.uniform(-1, 1, (c.height, k)).astype(np.float32))
+        b.set_x(i, xs[-1])
+        b.set_y(i, y0s[-1])
+    b.run(1)
+    _check_all(b, coos, xs, y0s, f"bsr k={k}")
+    eager = [b.get_y(i) for i in range(len(coos))]
+    st = b.stats()
+    assert st["nnz"] == sum(c.nnz for c in coos)
+    b.capture()
+    for rep in range(3):
+        for i in range(len(coos)):
+            b.set_y(i, y0s[i])
+        b.run(1)
+        for i in range(len(coos)):
+            assert np.array_equal(b.get_y(i).view(np.uint32), eager[i].view(np.uint32)), (rep, i)
+    b.free()
+
+
+def test_batch_bsr_refuses_k1_and_bad_blocks(ctx):
+    coo = M.gen_block_rows(50, 3, 4, 10, 17)
+    for mat, k in ((M.coo2bsr(coo, 3, 3), 1), (M.coo2bsr(coo, 3, 2), 4), (M.coo2bsr(coo, 5, 5), 4)):
+        with pytest.raises(L.ClspmvError):
+            B.Batch(ctx, [mat], [k])
+    assert b"batch" in L.lib().clspmv_last_error()

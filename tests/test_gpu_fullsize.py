@@ -1,9 +1,8 @@
 """GPU parity at the benchmark sizes: each single-format matrix (27-point
 stencil 128^3 as DIA, 4M x 32 uniform as ELL in the LDS x-tiled layout, 2M
 Zipf(1.8) rows as CSR x-tiled, 1M block rows of 3x3 blocks as sliced BSR)
-and the 4M-row Cocktail composite through its analyzed and uploaded
-partition, one y = y0 + A x each, checked element by element against the
-oracle (the reference's FP32 CPU SpMV restated, with the FP64 product;
+and the 4M-row Cocktail composite through the analyzer's MI355X plan, one
+y = y0 + A x each, checked element by element against the oracle (the reference's FP32 CPU SpMV restated, with the FP64 product;
 1e-5 relative bound, condition-aware: oracle.parity_report)."""
 import numpy as np
 import pytest
@@ -46,15 +45,16 @@ def test_single_format_full_size(ctx, name):
 def test_cocktail_composite_full_size(ctx):
     coo = W.make_coo("c2")
     print(f"c2: rows {coo.height}, nnz {coo.nnz}", flush=True)
-    mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(CK.MI355X_TABLES))
+    plan = CK.Plan.analyze(coo.copy(), CK.BenchTables(CK.MI355X_TABLES))
     x = M.gen_vector(coo.width, 7)
     y0 = M.gen_vector(coo.height, 8)
-    gpu = CK.GpuCocktail(ctx, mat, x, y0)
+    gpu = CK.GpuCocktail(ctx, plan, x, y0)
     kernels = [p["kernel"] for p in gpu.part_infos()]
+    assert len(kernels) == len(plan.parts()), kernels
     gpu.do_spmv(1)
     y = gpu.result_to_cpu()
     gpu.free()
-    mat.free()
+    plan.free()
     rep = O.parity_report(y, coo, x, y0)
     coo.free()
     assert rep["ok"], (kernels, rep)
