@@ -250,6 +250,8 @@ _SIGS = [
     ("clspmv_probe_launch", C.c_int, [C.c_void_p, C.c_int, _FP]),
     ("clspmv_probe_launch_grid", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _FP]),
     ("clspmv_probe_graph_launch", C.c_int, [C.c_void_p, C.c_int, C.c_int, _FP]),
+    ("clspmv_coo_spmv_noalloc", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]),
 ]
 
 _lib = None

@@ -13,7 +13,9 @@
 //      row (may continue into the next) go to a per-chunk carry pair in FP64;
 //   4. coo_carry_kernel adds each run of equal-row carries in order.
 // No atomics; results are deterministic.  Entries are padded to a multiple of
-// 512 with (last row, last col, 0.0), as the reference pads to 32.
+// 512 with (last row, last col, 0.0), as the reference pads to 32; arrays
+// that end earlier (nnz below the chunk multiple) are padded the same way
+// as the last chunk is loaded.
 #include "kcommon.hpp"
 #include "launch.hpp"
 
@@ -21,7 +23,7 @@ namespace clspmv {
 namespace dev {
 
 __global__ __launch_bounds__(256) void coo_segment_kernel(const int* __restrict__ rowi, const int* __restrict__ coli,
-                                                          const float* __restrict__ vali, int chunks,
+                                                          const float* __restrict__ vali, int nnz, int chunks,
                                                           const float* __restrict__ x, float* __restrict__ y,
                                                           int* __restrict__ carry_row,
                                                           double* __restrict__ carry_val) {
@@ -31,7 +33,19 @@ __global__ __launch_bounds__(256) void coo_segment_kernel(const int* __restrict_
     const size_t base = (size_t)chunk * kCooChunk + (size_t)lane * kCooPerLane;
     int r[8], c[8];
     float v[8];
-    {
+    if (base + kCooPerLane > (size_t)nnz) {
+        // past the end of unpadded arrays: (last row, last col, 0.0), every
+        // load at a valid index
+        const int last = nnz - 1;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const bool in = base + (size_t)i < (size_t)nnz;
+            const size_t k = in ? base + (size_t)i : (size_t)last;
+            r[i] = rowi[k];
+            c[i] = coli[k];
+            v[i] = in ? vali[k] : 0.f;
+        }
+    } else {
         const int4* rp4 = reinterpret_cast<const int4*>(rowi + base);
         const int4* cp4 = reinterpret_cast<const int4*>(coli + base);
         const float4* vp4 = reinterpret_cast<const float4*>(vali + base);
@@ -138,7 +152,7 @@ __global__ __launch_bounds__(256) void coo_carry_kernel(const int* __restrict__ 
 int launch_coo(const DevCoo& m, const float* x, float* y, hipStream_t st) {
     if (m.chunks == 0) return 0;
     hipLaunchKernelGGL(dev::coo_segment_kernel, dim3((m.chunks + 3) / 4), dim3(256), 0, st, m.row, m.col, m.val,
-                       m.chunks, x, y, m.carry_row, m.carry_val);
+                       m.nnz, m.chunks, x, y, m.carry_row, m.carry_val);
     int e = (int)hipGetLastError();
     if (e) return e;
     const int n = 2 * m.chunks;

@@ -56,6 +56,7 @@ struct DevCoo {
     const int* col;
     const float* val;
     int nnz_padded;           // multiple of the chunk size
+    int nnz;                  // entries the arrays hold (nnz_padded, or fewer: the last chunk is padded on the fly)
     int chunks;
     int rows;
     int* carry_row;           // 2 per chunk

@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -334,6 +335,43 @@ int finish_xtile(clspmv_part_t p, int fmt, int rows, int cols, long long nnz) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- allocation-free COO
+// clspmv_coo_spmv_noalloc keeps the COO kernels' chunk carries (two per
+// 512-entry chunk: an int row and an FP64 sum) in this array instead of a
+// device allocation: at most kNoAllocChunks chunks (nnz <= 167,772,160),
+// 7.5 MiB held by every device the library runs on until the process ends.
+// Host code uses the per-device address from hipGetSymbolAddress, never the
+// array's name (which in host code is only a host-side stand-in).
+namespace {
+constexpr int kNoAllocChunks = 327680;
+constexpr size_t kNoAllocRowBytes = (size_t)2 * kNoAllocChunks * sizeof(int);
+constexpr size_t kNoAllocBytes = kNoAllocRowBytes + (size_t)2 * kNoAllocChunks * sizeof(double);
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};  // device address of noalloc_scratch, per device
+}  // namespace
+
+static __device__ __align__(256) unsigned char noalloc_scratch[kNoAllocBytes];
+
+// Once per device, with that device current (called by clspmv_init).
+static int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device %d beyond the %d the library supports", device, kMaxDevices);
+        return CLSPMV_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return CLSPMV_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess) return hip_status(e, "hipGetSymbolAddress(noalloc_scratch)");
+    if (!ptr) {
+        set_error("hipGetSymbolAddress(noalloc_scratch) returned no address");
+        return CLSPMV_E_HIP;
+    }
+    g_noalloc_ptr[device] = ptr;
+    return CLSPMV_OK;
+}
+
 extern "C" {
 
 int clspmv_init(int device, clspmv_ctx_t* ctx) {
@@ -346,6 +384,7 @@ int clspmv_init(int device, clspmv_ctx_t* ctx) {
     }
     if (device < 0 || device >= n) return CLSPMV_E_INVALID;
     if ((e = hipSetDevice(device)) != hipSuccess) return hip_status(e, "hipSetDevice");
+    if (int rc = noalloc_setup(device)) return rc;
     auto* c = new (std::nothrow) clspmv_ctx_s();
     if (!c) return CLSPMV_E_NOMEM;
     c->device = device;
@@ -1207,6 +1246,7 @@ int clspmv_part_coo(clspmv_ctx_t ctx, const clspmv_coo_matrix* m, int method, cl
         p->coo.val = (const float*)p->add(ctx, vv.data(), padded * 4, &rc);
         if (rc) return rc;
         p->coo.nnz_padded = (int)padded;
+        p->coo.nnz = (int)padded;
         p->coo.chunks = chunks;
         p->coo.rows = rows;
         const size_t ncarry = (size_t)std::max(1, 2 * chunks);
@@ -1672,6 +1712,42 @@ int clspmv_part_free(clspmv_part_t p) {
     (void)hipStreamSynchronize(p->ctx->stream());
     delete p;
     return CLSPMV_OK;
+}
+
+// ---------------------------------------------------------------- allocation-free COO
+
+int clspmv_coo_spmv_noalloc(clspmv_ctx_t ctx, int rows, int nnz, const int* d_row, const int* d_col,
+                            const float* d_val, const float* d_x, float* d_y) {
+    if (!ctx || rows < 0 || nnz < 0) return CLSPMV_E_INVALID;
+    if (nnz == 0) return CLSPMV_OK;
+    if (!d_row || !d_col || !d_val || !d_x || !d_y || rows == 0) return CLSPMV_E_INVALID;
+    if (nnz > kNoAllocChunks * kCooChunk) {
+        set_error("clspmv_coo_spmv_noalloc: nnz %d above the %d its scratch holds", nnz, kNoAllocChunks * kCooChunk);
+        return CLSPMV_E_TOO_LARGE;
+    }
+    if ((reinterpret_cast<uintptr_t>(d_row) | reinterpret_cast<uintptr_t>(d_col) |
+         reinterpret_cast<uintptr_t>(d_val)) & 15) {
+        set_error("clspmv_coo_spmv_noalloc: row, col and val must be 16-byte aligned device pointers");
+        return CLSPMV_E_INVALID;
+    }
+    void* base = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_noalloc_ptr[ctx->device] : nullptr;
+    if (!base) {
+        set_error("clspmv_coo_spmv_noalloc: scratch of device %d not set up", ctx->device);
+        return CLSPMV_E_INVALID;
+    }
+    DevCoo m{};
+    m.row = d_row;
+    m.col = d_col;
+    m.val = d_val;
+    m.chunks = (nnz + kCooChunk - 1) / kCooChunk;
+    m.nnz_padded = m.chunks * kCooChunk;
+    m.nnz = nnz;
+    m.rows = rows;
+    // every carry slot of the m.chunks chunks is written by the segment
+    // kernel before the carry kernel reads it: nothing to clear
+    m.carry_row = static_cast<int*>(base);
+    m.carry_val = reinterpret_cast<double*>(static_cast<unsigned char*>(base) + kNoAllocRowBytes);
+    return hip_status((hipError_t)launch_coo(m, d_x, d_y, ctx->stream()), "coo_spmv_noalloc launch");
 }
 
 // ---------------------------------------------------------------- probes

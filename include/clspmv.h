@@ -399,6 +399,20 @@ int clspmv_set_option(const char* name, long long value);
 int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info);
 int clspmv_part_free(clspmv_part_t p);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): y += A x for a COO matrix held in the
+ * caller's device arrays -- sorted by row, indices in range, row/col/val
+ * 16-byte aligned (as clspmv_part_coo requires and checks at upload) --
+ * with the kernels of clspmv_part_coo method 0.  It allocates nothing: the
+ * chunk carries live in a static device buffer of the library (7.5 MiB per
+ * device, held until the process ends), so nnz <= 167,772,160 (else
+ * CLSPMV_E_TOO_LARGE).  Every call on one device shares that buffer: calls
+ * that overlap on one device (other streams, other threads) race.
+ * Asynchronous on the context stream. */
+int clspmv_coo_spmv_noalloc(clspmv_ctx_t ctx, int rows, int nnz, const int* d_row, const int* d_col,
+                            const float* d_val, const float* d_x, float* d_y);
+
 /* ------------------------------------------------------------------ */
 /* Bandwidth / launch-overhead probes: bandwidth_test and bench_overhead
  * (A/src/mem_bandwidth.cpp:14-579, A/src/bench_overhead.cpp:14-184). */
