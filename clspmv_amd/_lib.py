@@ -250,6 +250,7 @@ _SIGS = [
     ("clspmv_probe_launch", C.c_int, [C.c_void_p, C.c_int, _FP]),
     ("clspmv_probe_launch_grid", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _FP]),
     ("clspmv_probe_graph_launch", C.c_int, [C.c_void_p, C.c_int, C.c_int, _FP]),
+    ("clspmv_part_xtile_stats", C.c_int, [C.c_void_p, _P(C.c_longlong), _P(C.c_longlong)]),
     ("clspmv_coo_spmv_noalloc", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p]),
 ]

@@ -46,10 +46,13 @@ namespace {
 
 constexpr int kXtile = 4;  // method number of the x-tiled layout
 
-// do_spmv and power_iterate replay captured hipGraphs of up to kGraphPasses
-// passes (one pass = every part's launch in order) instead of launching
-// each part from the host; "cocktail_graph" 0 launches directly.
-bool g_use_graph = true;
+// do_spmv and power_iterate launch each part from the host by default; with
+// "cocktail_graph" 1 they replay captured hipGraphs of up to kGraphPasses
+// passes (one pass = every part's launch in order).  On MI355X with ROCm
+// 7.2 a graph replay costs about 17-27 us per launch and 1.65 us per node at
+// 64 nodes, against 1.5-1.65 us per plain launch, so replaying saves nothing
+// and costs ~20 us when a call runs few passes.
+bool g_use_graph = false;
 constexpr int kGraphPasses = 64;
 
 // x[i] = y[i] for i < n: the vector update of one iteration.

@@ -78,6 +78,7 @@ struct clspmv_part_s {
     DevCsrSliced csrs{};
     long long carry_bytes = 0;
     double codec_bound = 0.0;  // x-tiled encoder's bound, copied into info by finish_xtile
+    long long xt_tiles = 0, xt_staged_bytes = 0;  // x-tiled: tiles and x bytes staged per SpMV
     int csr_lanes = 0;
     DevCoo coo{};
     DevDia dia{};
@@ -722,6 +723,7 @@ int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int
 }
 
 int g_xt_blocks = 256;  // target workgroups: one per CU (tunable: "xtile_blocks")
+
 // Column groups per row block at most (tunable: "xtile_max_groups", 1..8).
 int g_xt_max_groups = kXtMaxGroups;
 
@@ -846,6 +848,13 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
     if (rc) return rc;
     d.tile_chunk = (const int*)p->add(ctx, tile_chunk.data(), tile_chunk.size() * 4, &rc);
     if (rc) return rc;
+    {
+        // x bytes staged per SpMV: one slice for each tile that starts one
+        long long starts = 0;
+        for (size_t t = 0; t < tile_slice.size(); ++t) starts += !(tile_slice[t] & 1);
+        p->xt_tiles = (long long)tile_slice.size();
+        p->xt_staged_bytes = starts * (long long)kXtCols * 4;
+    }
     if (codec == 0) {
         // FP32: per chunk and lane {entry 2l, entry 2l+1, value 2l, value
         // 2l+1}, one 16-byte load per lane (val is not used)
@@ -1698,6 +1707,13 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes) 
         int e = clspmv_part_launch(p, d_x, d_y, st);
         if (e) return hip_status((hipError_t)e, "kernel launch");
     }
+    return CLSPMV_OK;
+}
+
+int clspmv_part_xtile_stats(clspmv_part_t p, long long* tiles, long long* staged_x_bytes) {
+    if (!p) return CLSPMV_E_INVALID;
+    if (tiles) *tiles = p->xt_tiles;
+    if (staged_x_bytes) *staged_x_bytes = p->xt_staged_bytes;
     return CLSPMV_OK;
 }
 

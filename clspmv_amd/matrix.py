@@ -430,6 +430,12 @@ class Part:
         """Enqueue ntimes * (y += A x) on the context stream."""
         check(lib().clspmv_part_spmv(self.h, x.ptr, y.ptr, ntimes), "part_spmv")
 
+    def xtile_stats(self):
+        """LDS x-tiled parts: tiles and staged x bytes per SpMV (0 otherwise)."""
+        t, b = C.c_longlong(), C.c_longlong()
+        check(lib().clspmv_part_xtile_stats(self.h, C.byref(t), C.byref(b)), "part_xtile_stats")
+        return {"tiles": t.value, "staged_x_bytes": b.value}
+
     def free(self):
         if self.h and self.ctx.h:
             lib().clspmv_part_free(self.h)

@@ -393,10 +393,14 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  *                       (default 100000 = 1e-4; 0 = no guard): a run part
  *                       whose codec_bound (clspmv_part_info) exceeds it is
  *                       built in FP32 instead;
- *   "cocktail_graph"    1 (default): do_spmv and power_iterate replay
- *                       captured hipGraphs; 0: direct launches. */
+ *   "cocktail_graph"    0 (default): do_spmv and power_iterate launch each
+ *                       part directly; 1: they replay captured hipGraphs. */
 int clspmv_set_option(const char* name, long long value);
 int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info);
+/* LDS x-tiled parts (no reference counterpart): tiles per SpMV (one x
+ * slice of one row block's column group each) and bytes of x staged into
+ * LDS per SpMV (one slice per tile that starts a slice); 0 for other parts. */
+int clspmv_part_xtile_stats(clspmv_part_t p, long long* tiles, long long* staged_x_bytes);
 int clspmv_part_free(clspmv_part_t p);
 
 /* No counterpart in the reference.  For callers that cannot allocate

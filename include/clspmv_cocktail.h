@@ -108,14 +108,16 @@ int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* mat, const 
 int clspmv_free_mat_kernels(clspmv_gpumat_t gpumat);
 /* do_spmv (:1088-1132): ntimes * (y += A x), parts in the reference's fixed
  * order BDIA, DIA, SBELL, BELL, BCSR, SELL, ELL, CSR, COO on one stream;
- * asynchronous (clspmv_synchronize waits).  The passes are replayed from
+ * asynchronous (clspmv_synchronize waits).  With
+ * clspmv_set_option("cocktail_graph", 1) the passes are replayed from
  * hipGraphs captured on first use (up to 64 passes per graph launch);
- * clspmv_set_option("cocktail_graph", 0) launches each part directly. */
+ * by default each part is launched directly (measured no slower). */
 int clspmv_do_spmv(clspmv_gpumat_t gpumat, int ntimes);
 /* The iteration of the reference's example (A/src/eval.cpp:19-72, with
  * cpy_vector_from_gpu :1510-1556): `iterations` times { y += A x; x := y },
- * all on the device, from captured graphs of up to 64 iterations.  Square
- * matrices only; asynchronous. */
+ * all enqueued on the device with no host round trip (x := y by a copy
+ * kernel; from captured graphs of up to 64 iterations with
+ * "cocktail_graph" 1).  Square matrices only; asynchronous. */
 int clspmv_power_iterate(clspmv_gpumat_t gpumat, int iterations);
 /* cpy_vector_from_cpu / _from_gpu, cpy_result_from_cpu / _from_gpu /
  * _to_cpu / _to_gpu (:1458-1588).  *_gpu take device pointers. */

@@ -175,10 +175,10 @@ def test_power_iteration(ctx, tables):
 
 
 def test_power_iteration_device_equals_host_loop(ctx, tables):
-    # the device-resident iteration (captured graphs, x := y by a kernel)
-    # takes bitwise the same steps as the reference's host-driven loop
-    # (do_spmv, then cpy_vector_from_gpu), also across graph sizes (70
-    # iterations = a 64-iteration graph and a 6-iteration graph)
+    # the device-resident iteration (x := y by a kernel; direct launches or
+    # captured graphs) takes bitwise the same steps as the reference's
+    # host-driven loop (do_spmv, then cpy_vector_from_gpu), also across graph
+    # sizes (70 iterations = a 64-iteration graph and a 6-iteration graph)
     coo = AC.make(GEN["cocktail_64k"])
     vals = (coo.data * np.float32(0.05)).astype(np.float32)
     coo = M.CooMatrix.from_arrays(coo.row_id, coo.col_id, vals, coo.width, coo.height)
@@ -186,14 +186,19 @@ def test_power_iteration_device_equals_host_loop(ctx, tables):
     x = M.gen_vector(coo.width, 31)
     y0 = np.zeros(coo.height, np.float32)
     out = {}
-    for k in (3, 70):
-        a = CK.GpuCocktail(ctx, mat, x, y0)
-        b = CK.GpuCocktail(ctx, mat, x, y0)
-        out["dev", k] = CK.power_iterate(ctx, a, k)
-        out["host", k] = CK.power_iterate_host(ctx, b, k)
-        assert np.array_equal(out["dev", k].view(np.uint32), out["host", k].view(np.uint32)), k
-        a.free()
-        b.free()
+    for graph in (0, 1):
+        L.check(L.lib().clspmv_set_option(b"cocktail_graph", graph), "set_option")
+        try:
+            for k in (3, 70):
+                a = CK.GpuCocktail(ctx, mat, x, y0)
+                b = CK.GpuCocktail(ctx, mat, x, y0)
+                out["dev", k] = CK.power_iterate(ctx, a, k)
+                out["host", k] = CK.power_iterate_host(ctx, b, k)
+                assert np.array_equal(out["dev", k].view(np.uint32), out["host", k].view(np.uint32)), (graph, k)
+                a.free()
+                b.free()
+        finally:
+            L.check(L.lib().clspmv_set_option(b"cocktail_graph", 0), "set_option")
     # the first step is y = 0 + A x: the oracle's product
     g = CK.GpuCocktail(ctx, mat, x, y0)
     rep = O.parity_report(CK.power_iterate(ctx, g, 1), coo, x, y0)
@@ -212,20 +217,20 @@ def test_graph_replays_equal_direct_launches(ctx, tables):
     g = CK.GpuCocktail(ctx, mat, x, y0)
     assert len(g.part_infos()) >= 2
     ref = []
-    L.check(L.lib().clspmv_set_option(b"cocktail_graph", 0), "set_option")
+    for n in (1, 2, 65):
+        g.result_from_cpu(y0)
+        g.do_spmv(n)
+        ref.append(g.result_to_cpu())
+    L.check(L.lib().clspmv_set_option(b"cocktail_graph", 1), "set_option")
     try:
-        for n in (1, 2, 65):
-            g.result_from_cpu(y0)
-            g.do_spmv(n)
-            ref.append(g.result_to_cpu())
+        for rep in range(3):
+            for n, r in zip((1, 2, 65), ref):
+                g.result_from_cpu(y0)
+                g.do_spmv(n)
+                ctx.sync()
+                assert np.array_equal(g.result_to_cpu().view(np.uint32), r.view(np.uint32)), (rep, n)
     finally:
-        L.check(L.lib().clspmv_set_option(b"cocktail_graph", 1), "set_option")
-    for rep in range(3):
-        for n, r in zip((1, 2, 65), ref):
-            g.result_from_cpu(y0)
-            g.do_spmv(n)
-            ctx.sync()
-            assert np.array_equal(g.result_to_cpu().view(np.uint32), r.view(np.uint32)), (rep, n)
+        L.check(L.lib().clspmv_set_option(b"cocktail_graph", 0), "set_option")
     rep1 = O.parity_report(ref[0], coo, x, y0)
     assert rep1["ok"], rep1
     g.free()
