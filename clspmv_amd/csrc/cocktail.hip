@@ -262,11 +262,23 @@ int add_flat_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& m, int meth, int 
 // a single run: the caller then keeps the partition's own flat part.
 constexpr int kOneRun = 1;
 
-// Row-run classes.  Local runs are refined by shape: rows of dense aligned
-// b x b blocks (b = 4 or 2; >= 95 % of the entries in full blocks) run as
-// native sliced BSR, rows of near-equal length (SELL with 32-row slices
-// pads <= 25 %) as SELL, other rows as CSR row blocks.
-enum RunClass : signed char { kScattered = 0, kLocalCsr = 1, kLocalSell = 2, kLocalBsr2 = 12, kLocalBsr4 = 14 };
+// Row-run classes.  Local runs are refined by shape: rows whose entries lie
+// on at most 64 diagonals that they fill to >= 2/3 run as DIA (4 bytes per
+// slot, x read along the diagonals); rows of dense aligned b x b blocks
+// (b = 4, 3 or 2; >= 95 % of the entries in full blocks) as native sliced
+// BSR; rows of near-equal length (SELL with 32-row slices pads <= 25 %) as
+// SELL; other rows as CSR row blocks.  The classes that change the layout
+// of a single-run matrix (DIA, BSR) are the ones >= kLocalDia.
+enum RunClass : signed char {
+    kScattered = 0,
+    kLocalCsr = 1,
+    kLocalSell = 2,
+    kLocalDia = 10,
+    kLocalBsr2 = 12,
+    kLocalBsr3 = 13,
+    kLocalBsr4 = 14
+};
+constexpr int kDiaMaxDiagonals = 64;
 
 // Share of a row block's entries in full b x b blocks (block rows counted
 // from r0, which is a multiple of b).
@@ -308,6 +320,25 @@ signed char classify_block(const clspmv_csr_matrix& m, const std::vector<int>& r
                 repeats = true;
                 break;
             }
+    if (!repeats) {
+        // diagonals: distinct column - row offsets (at most 64, filled to 2/3)
+        std::vector<long long> off;
+        off.reserve((size_t)n);
+        bool few = true;
+        for (int r = r0; r < r1 && few; ++r)
+            for (int k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) {
+                off.push_back((long long)m.csr_col_id[k] - r);
+                if (off.size() % 4096 == 0) {  // keep the distinct ones, stop early past the limit
+                    std::sort(off.begin(), off.end());
+                    off.erase(std::unique(off.begin(), off.end()), off.end());
+                    if ((int)off.size() > kDiaMaxDiagonals) few = false;
+                }
+            }
+        std::sort(off.begin(), off.end());
+        off.erase(std::unique(off.begin(), off.end()), off.end());
+        if (few && (int)off.size() <= kDiaMaxDiagonals && 3 * n >= 2LL * (long long)off.size() * (r1 - r0))
+            return kLocalDia;
+    }
     if (!repeats && full_block_share(m, rp, r0, r1, 4) >= 0.95) return kLocalBsr4;
     if (!repeats && full_block_share(m, rp, r0, r1, 2) >= 0.95) return kLocalBsr2;
     long long slots = 0;
@@ -325,8 +356,10 @@ int run_method(signed char cls, int rows, int cols, long long nnz) {
     return cls == kLocalCsr ? 1 : 0;
 }
 int run_format(signed char cls) {
-    return cls == kLocalSell ? CLSPMV_FMT_SELL : (cls == kLocalBsr2 || cls == kLocalBsr4) ? CLSPMV_FMT_BSR
-                                                                                          : CLSPMV_FMT_CSR_VECTOR;
+    return cls == kLocalSell                                             ? CLSPMV_FMT_SELL
+           : cls == kLocalDia                                            ? CLSPMV_FMT_DIA
+           : (cls == kLocalBsr2 || cls == kLocalBsr3 || cls == kLocalBsr4) ? CLSPMV_FMT_BSR
+                                                                           : CLSPMV_FMT_CSR_VECTOR;
 }
 
 // One run as a part of its class and method; rows [0, sub.height) of `sub`.
@@ -359,8 +392,13 @@ int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char c
         rc = clspmv_coo2sell(&coo, &sell, 32);
         if (!rc) rc = clspmv_part_sell(g->ctx, &sell, 0, &p);
         clspmv_free_sell(&sell);
+    } else if (cls == kLocalDia) {
+        clspmv_dia_matrix dia{};
+        rc = clspmv_coo2dia(&coo, &dia, 32);
+        if (!rc) rc = clspmv_part_dia(g->ctx, &dia, 0, &p);
+        clspmv_free_dia(&dia);
     } else {
-        const unsigned b = cls == kLocalBsr4 ? 4 : 2;
+        const unsigned b = cls == kLocalBsr4 ? 4 : cls == kLocalBsr3 ? 3 : 2;
         clspmv_bsr_matrix bsr{};
         rc = clspmv_coo2bsr(&coo, &bsr, b, b);
         if (!rc && g_run_codec) {
@@ -429,6 +467,18 @@ int plan_runs(clspmv_plan_s* P, bool single_ok) {
         push_run(kScattered, 0, rows);  // one CSR part, by the cost rule (local x: adaptive rows)
         return CLSPMV_OK;
     }
+    // 3 x 3 blocks do not align with 32768-row runs: a matrix made of them
+    // (its first run local and >= 95 % of all entries in full blocks counted
+    // from row 0) runs whole as sliced BSR 3 x 3
+    {
+        const int r1 = std::min(rows, kRunBlock);
+        const signed char c0 = classify_block(merged, rp, 0, r1);
+        if (c0 > kScattered && c0 < kLocalDia && full_block_share(merged, rp, 0, r1, 3) >= 0.95 &&
+            full_block_share(merged, rp, 0, rows, 3) >= 0.95) {
+            push_run(kLocalBsr3, 0, rows);
+            return CLSPMV_OK;
+        }
+    }
     std::vector<signed char> cls((size_t)nbk);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(omp_threads())
     for (int b = 0; b < nbk; ++b)
@@ -439,10 +489,10 @@ int plan_runs(clspmv_plan_s* P, bool single_ok) {
     for (int b = 1; b < nbk; ++b)
         if (cls[(size_t)b] != cls[(size_t)b - 1]) cut.push_back(b * kRunBlock);
     cut.push_back(rows);
-    if (cut.size() == 2 && cls[0] < kLocalBsr2) {
-        // one run: only a block-structured one changes layout (to sliced
-        // BSR); otherwise the partition's own flat parts (single_ok false)
-        // or the merged rows as one CSR part
+    if (cut.size() == 2 && cls[0] < kLocalDia) {
+        // one run: only a band or block-structured one changes layout (to
+        // DIA or sliced BSR); otherwise the partition's own flat parts
+        // (single_ok false) or the merged rows as one CSR part
         if (!single_ok) return kOneRun;
         push_run(kScattered, 0, rows);
         return CLSPMV_OK;

@@ -58,8 +58,27 @@ def cocktail_times(ctx, idx, seed, log2_min, log2_max):
     return out
 
 
+def eval_cocktail(ctx, data_dir, n, subset, seed, log2_min, log2_max, test_frac=0.2):
+    """Regret of the Cocktail plan (MI355X tables) on the first `subset`
+    held-out matrices of a saved dataset, against its autotuned times."""
+    d = np.load(os.path.join(data_dir, "dataset.npz"))
+    times = d["times"]
+    rng = np.random.default_rng(seed + 1)
+    perm = rng.permutation(n)
+    test = np.sort(perm[:int(n * test_frac)])[:subset]
+    ck = cocktail_times(ctx, test, seed, log2_min, log2_max)
+    r = np.array([ck[int(i)] / times[i].min() - 1.0 for i in test])
+    kinds = d["kinds"][test]
+    out = {"all": summary(r)}
+    for k in sorted(set(kinds.tolist())):
+        out[k] = summary(r[kinds == k])
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
+    ap.add_argument("--eval-cocktail", metavar="DIR", default=None,
+                    help="only re-time the Cocktail plan on the held-out subset of the dataset in DIR")
     ap.add_argument("--n", type=int, default=4000)
     ap.add_argument("--test-frac", type=float, default=0.2)
     ap.add_argument("--seed", type=int, default=0)
@@ -67,10 +86,18 @@ def main(argv=None):
     ap.add_argument("--log2-max", type=float, default=18)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--cocktail-subset", type=int, default=200)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
-    os.makedirs(args.out, exist_ok=True)
     ctx = M.Context(0)
+    if args.eval_cocktail:
+        res = eval_cocktail(ctx, args.eval_cocktail, args.n, args.cocktail_subset, args.seed, args.log2_min,
+                            args.log2_max, args.test_frac)
+        print(json.dumps({"cocktail_mi355x_tables": res}, indent=1), flush=True)
+        ctx.close()
+        return
+    if not args.out:
+        ap.error("--out is required")
+    os.makedirs(args.out, exist_ok=True)
     t0 = time.perf_counter()
     feats, times, kinds, nnz = S.build_dataset(ctx, args.n, args.seed, log, args.log2_min, args.log2_max)
     np.savez(os.path.join(args.out, "dataset.npz"), features=feats, times=times, kinds=kinds, nnz=nnz,

@@ -277,7 +277,8 @@ def test_probes(ctx):
     dst.free()
 
 
-@pytest.mark.parametrize("case", ["cocktail_64k", "band15_rand8", "blk4_rand", "zipf_20k"])
+@pytest.mark.parametrize("case", ["cocktail_64k", "band15_rand8", "blk4_rand", "zipf_20k", "st27_24", "blk3",
+                                  "lap2d_256"])
 @pytest.mark.parametrize("tname", ["reference", "mi355x"])
 def test_plan_kernels(ctx, case, tname):
     # the analyzer's MI355X plan uploaded as planned: one device part per

@@ -66,3 +66,13 @@ def test_plan_of_a_callers_partition_equals_analyze_plan(tables):
     a = CK.Plan.of(mat).parts()
     b = CK.Plan.analyze(coo.copy(), tables["mi355x"]).parts()
     assert a == b
+
+
+def test_band_and_block_runs(tables):
+    # with the MI355X tables the analyzer leaves stencils and 3 x 3 block
+    # matrices in its flat parts; the plan runs their rows as DIA (27 or 5
+    # diagonals, filled) and as whole-matrix sliced BSR 3 x 3
+    for case, fmt in (("st27_24", L.FMT_DIA), ("st27_40", L.FMT_DIA), ("lap2d_64", L.FMT_DIA), ("blk3", L.FMT_BSR)):
+        coo = AC.make(dict(AC.CASES)[case])
+        parts = CK.Plan.analyze(coo.copy(), tables["mi355x"]).parts()
+        assert [(q["source"], q["format"], q["row0"], q["row1"]) for q in parts] == [(9, fmt, 0, coo.height)], case
