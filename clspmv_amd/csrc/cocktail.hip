@@ -259,7 +259,8 @@ int add_flat_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& m, int meth, int 
 }
 
 // Returns kOneRun (nothing added) when single_ok is false and the rows form
-// a single run: the caller then keeps the partition's own flat part.
+// a single run of near-equal rows: the caller then keeps the partition's
+// own flat part.
 constexpr int kOneRun = 1;
 
 // Row-run classes.  Local runs are refined by shape: rows whose entries lie
@@ -461,17 +462,21 @@ int plan_runs(clspmv_plan_s* P, bool single_ok) {
         const long long n = rp[(size_t)r1] - rp[(size_t)r0];
         P->parts.push_back(PlanPart{9, cls, run_format(cls), run_method(cls, r1 - r0, cols, n), r0, r1, n});
     };
-    const int nbk = (rows + kRunBlock - 1) / kRunBlock;
+    // run blocks: 32768 rows, finer (down to 4096) below 512K rows so that a
+    // small matrix still has ~16 blocks to tell its regions apart
+    int rb = 4096;
+    while (rb < kRunBlock && rb * 32 <= rows) rb *= 2;
+    const int nbk = (rows + rb - 1) / rb;
     if (!g_split_local || nbk == 0) {
         if (!single_ok) return kOneRun;
         push_run(kScattered, 0, rows);  // one CSR part, by the cost rule (local x: adaptive rows)
         return CLSPMV_OK;
     }
-    // 3 x 3 blocks do not align with 32768-row runs: a matrix made of them
-    // (its first run local and >= 95 % of all entries in full blocks counted
-    // from row 0) runs whole as sliced BSR 3 x 3
+    // 3 x 3 blocks do not align with power-of-two runs: a matrix made of
+    // them (its first run local and >= 95 % of all entries in full blocks
+    // counted from row 0) runs whole as sliced BSR 3 x 3
     {
-        const int r1 = std::min(rows, kRunBlock);
+        const int r1 = std::min(rows, rb);
         const signed char c0 = classify_block(merged, rp, 0, r1);
         if (c0 > kScattered && c0 < kLocalDia && full_block_share(merged, rp, 0, r1, 3) >= 0.95 &&
             full_block_share(merged, rp, 0, rows, 3) >= 0.95) {
@@ -482,22 +487,19 @@ int plan_runs(clspmv_plan_s* P, bool single_ok) {
     std::vector<signed char> cls((size_t)nbk);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(omp_threads())
     for (int b = 0; b < nbk; ++b)
-        cls[(size_t)b] = classify_block(merged, rp, b * kRunBlock, std::min(rows, (b + 1) * kRunBlock));
+        cls[(size_t)b] = classify_block(merged, rp, b * rb, std::min(rows, (b + 1) * rb));
     for (int b = 0; b < nbk; ++b)  // empty blocks join the previous run
         if (cls[(size_t)b] < 0) cls[(size_t)b] = b ? cls[(size_t)b - 1] : kLocalCsr;
     std::vector<int> cut{0};
     for (int b = 1; b < nbk; ++b)
-        if (cls[(size_t)b] != cls[(size_t)b - 1]) cut.push_back(b * kRunBlock);
+        if (cls[(size_t)b] != cls[(size_t)b - 1]) cut.push_back(b * rb);
     cut.push_back(rows);
-    if (cut.size() == 2 && cls[0] < kLocalDia) {
-        // one run: only a band or block-structured one changes layout (to
-        // DIA or sliced BSR); otherwise the partition's own flat parts
-        // (single_ok false) or the merged rows as one CSR part
-        if (!single_ok) return kOneRun;
-        push_run(kScattered, 0, rows);
-        return CLSPMV_OK;
+    if (cut.size() == 2 && cls[0] == kLocalSell && !single_ok) {
+        // one run of near-equal rows: the partition's own flat part (its
+        // ELL or SELL serves such rows as well as a SELL run)
+        return kOneRun;
     }
-    for (size_t i = 0; i + 1 < cut.size(); ++i) push_run(cls[(size_t)(cut[i] / kRunBlock)], cut[i], cut[i + 1]);
+    for (size_t i = 0; i + 1 < cut.size(); ++i) push_run(cls[(size_t)(cut[i] / rb)], cut[i], cut[i + 1]);
     return CLSPMV_OK;
 }
 

@@ -162,6 +162,14 @@ CANDIDATES = [
     ("bsr2_sliced", L.FMT_BSR, 0, "bsr2"),
     ("bsr3_sliced", L.FMT_BSR, 0, "bsr3"),
     ("bsr4_sliced", L.FMT_BSR, 0, "bsr4"),
+    # the reference's own tuning sweeps (A/src/bench_sell.cpp: slice heights
+    # 32 and 128; A/src/bench_sbell.cpp, bench_bcsr.cpp: block shapes)
+    ("sell32", L.FMT_SELL, 0, "sell32"),
+    ("sell128", L.FMT_SELL, 0, "sell128"),
+    ("sbell_1x4", L.FMT_SBELL, 0, "sbell14"),
+    ("sbell_2x4", L.FMT_SBELL, 0, "sbell24"),
+    ("sbell_4x4", L.FMT_SBELL, 0, "sbell44"),
+    ("bcsr_4x4", L.FMT_BCSR, 0, "bcsr44"),
 ]
 CANDIDATE_NAMES = [c[0] for c in CANDIDATES]
 DEFAULT = CANDIDATE_NAMES.index("csr_adaptive")
@@ -171,8 +179,9 @@ def applicable(feats):
     """Candidates worth converting to, from the features alone: DIA/BDIA when
     the diagonal padding stays below 2x the nonzeros (coo2dia refuses at 2x),
     ELL when its padding stays below 4x, BSR b x b when the blocks are at
-    least half full.  Row-list kernels always apply (x-tiled needs >= 4
-    columns)."""
+    least half full, SELL when ELL would pad less than 64x, the reference's
+    SBELL / BCSR block shapes when 4 x 4 blocks are at least half full.
+    Row-list kernels always apply (x-tiled needs >= 4 columns)."""
     f = np.atleast_2d(feats)
     ok = np.ones((f.shape[0], len(CANDIDATES)), bool)
     for i, (_, _, _, key) in enumerate(CANDIDATES):
@@ -182,6 +191,10 @@ def applicable(feats):
             ok[:, i] = f[:, 9] <= 2.0
         elif key.startswith("bsr"):
             ok[:, i] = f[:, 16 + (2, 3, 4).index(int(key[3]))] >= 0.5
+        elif key.startswith("sell"):
+            ok[:, i] = f[:, 9] <= 6.0
+        elif key.startswith(("sbell", "bcsr")):
+            ok[:, i] = f[:, 18] >= 0.5
     xt = [i for i, c in enumerate(CANDIDATES) if c[2] == 4]
     ok[:, xt] &= (f[:, 1:2] >= 2.0)  # log2(cols) >= 2
     return ok
@@ -203,6 +216,12 @@ def _convert(key, coo):
         if key.startswith("bsr"):
             b = int(key[3])
             return M.coo2bsr(coo, b, b)
+        if key.startswith("sell"):
+            return M.coo2sell(coo, int(key[4:]))
+        if key.startswith("sbell"):
+            return M.coo2sbell(coo, int(key[6]), int(key[5]), 32)
+        if key.startswith("bcsr"):
+            return M.coo2b4csr(coo, int(key[5]), int(key[4]))
     except L.ClspmvError:
         return None
     raise KeyError(key)

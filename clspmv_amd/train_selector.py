@@ -35,9 +35,10 @@ def summary(r):
             "within_10pct": float(np.mean(r <= 0.10)), "n": int(len(r))}
 
 
-def cocktail_times(ctx, idx, seed, log2_min, log2_max):
-    """Cocktail (MI355X tables, merged flat parts) time of matrices `idx` of
-    the dataset family."""
+def cocktail_times(ctx, idx, seed, log2_min, log2_max, detail=None, target_ms=1.0):
+    """Cocktail (MI355X tables, the analyzer's plan) time of matrices `idx`
+    of the dataset family, timed like the autotuner's candidates.  With
+    `detail` (a dict), also records per matrix its kind, size and kernels."""
     tables = CK.BenchTables(CK.MI355X_TABLES)
     rng = np.random.default_rng(seed)
     out = {}
@@ -50,24 +51,40 @@ def cocktail_times(ctx, idx, seed, log2_min, log2_max):
             gpu.do_spmv(1)
             ctx.sync()
             ctx.time_begin()
-            gpu.do_spmv(5)
-            out[i] = ctx.time_end() / 5
+            gpu.do_spmv(2)
+            ms = ctx.time_end() / 2
+            n = int(min(50, max(3, target_ms / max(ms, 1e-4))))
+            ctx.time_begin()
+            gpu.do_spmv(n)
+            out[i] = ctx.time_end() / n
+            if detail is not None:
+                detail[i] = {"kind": kind, "rows": int(coo.height), "nnz": int(coo.nnz),
+                             "kernels": [p["kernel"] for p in gpu.part_infos()], "ms": out[i]}
             gpu.free()
             mat.free()
         coo.free()
     return out
 
 
-def eval_cocktail(ctx, data_dir, n, subset, seed, log2_min, log2_max, test_frac=0.2):
+def eval_cocktail(ctx, data_dir, n, subset, seed, log2_min, log2_max, test_frac=0.2, detail=None):
     """Regret of the Cocktail plan (MI355X tables) on the first `subset`
-    held-out matrices of a saved dataset, against its autotuned times."""
+    held-out matrices of a saved dataset, against its autotuned times.
+    With `detail` (a list), appends one record per matrix: its plan's
+    kernels and time next to the best candidate's."""
     d = np.load(os.path.join(data_dir, "dataset.npz"))
     times = d["times"]
+    names = [str(c) for c in d["candidates"]]
     rng = np.random.default_rng(seed + 1)
     perm = rng.permutation(n)
     test = np.sort(perm[:int(n * test_frac)])[:subset]
-    ck = cocktail_times(ctx, test, seed, log2_min, log2_max)
+    rec = {}
+    ck = cocktail_times(ctx, test, seed, log2_min, log2_max, rec)
     r = np.array([ck[int(i)] / times[i].min() - 1.0 for i in test])
+    if detail is not None:
+        for i in test:
+            b = int(np.argmin(times[i]))
+            detail.append(dict(rec[int(i)], index=int(i), best=names[b], best_ms=float(times[i][b]),
+                               regret=float(ck[int(i)] / times[i][b] - 1.0)))
     kinds = d["kinds"][test]
     out = {"all": summary(r)}
     for k in sorted(set(kinds.tolist())):
@@ -87,11 +104,16 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--cocktail-subset", type=int, default=200)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--detail", default=None, help="with --eval-cocktail: write per-matrix records to this JSON file")
     args = ap.parse_args(argv)
     ctx = M.Context(0)
     if args.eval_cocktail:
+        detail = [] if args.detail else None
         res = eval_cocktail(ctx, args.eval_cocktail, args.n, args.cocktail_subset, args.seed, args.log2_min,
-                            args.log2_max, args.test_frac)
+                            args.log2_max, args.test_frac, detail)
+        if args.detail:
+            with open(args.detail, "w") as f:
+                json.dump(detail, f, indent=0)
         print(json.dumps({"cocktail_mi355x_tables": res}, indent=1), flush=True)
         ctx.close()
         return
