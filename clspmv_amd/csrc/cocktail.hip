@@ -7,7 +7,9 @@
 // as the reference pads them (x to a multiple of 8, y to a multiple of 1024,
 // A/src/spmv_cocktail.cpp:230-250).  do_spmv enqueues the parts in the
 // reference's fixed order on one stream, so parts that accumulate into the
-// same y are serialised as on the reference's in-order queue.
+// same y are serialised as on the reference's in-order queue; the row runs
+// of a plan, which write rows of their own, run side by side on up to four
+// streams (g_streams).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,7 +36,15 @@ struct clspmv_gpumat_s {
     // captured graphs: key = passes per graph * 2 + (1: each pass followed
     // by x := y, the iteration of A/src/eval.cpp:19-72)
     std::map<int, hipGraphExec_t> graphs;
+    // every part covers rows of its own (row runs of the plan): parts may
+    // then run side by side on the extra streams below
+    bool disjoint = false;
+    std::vector<hipStream_t> aux;  // created on first concurrent call
+    std::vector<hipEvent_t> ev;    // ev[0]: fork; ev[1 + a]: join of aux[a]
     ~clspmv_gpumat_s() {
+        for (auto s : aux) (void)hipStreamSynchronize(s);
+        for (auto s : aux) (void)hipStreamDestroy(s);
+        for (auto e : ev) (void)hipEventDestroy(e);
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         for (auto p : parts) clspmv_part_free(p);
         x.release();
@@ -54,6 +64,18 @@ constexpr int kXtile = 4;  // method number of the x-tiled layout
 // and costs ~20 us when a call runs few passes.
 bool g_use_graph = false;
 constexpr int kGraphPasses = 64;
+
+// do_spmv runs the parts of a plan made only of row runs (disjoint rows of
+// y) side by side: part j on stream j mod kStreams (stream 0 is the
+// context's, the others default-priority streams of the object), forked
+// from and joined back to the context stream by events once per call.
+// Every pass of a part stays on its stream, in order, and the parts only
+// read x, so passes need no join.  A small matrix's parts are each too
+// short to fill the GPU, and one after another their launch latencies
+// add up.  Switched off with clspmv_set_option("cocktail_streams", 0);
+// power_iterate and graph replay stay on one stream.
+bool g_streams = true;
+constexpr int kStreams = 4;
 
 // x[i] = y[i] for i < n: the vector update of one iteration.
 __global__ void copy_vec_kernel(const float* __restrict__ src, float* __restrict__ dst, int n) {
@@ -114,10 +136,45 @@ int pass_graph(clspmv_gpumat_s* g, int passes, bool iterate, hipGraphExec_t* out
     return CLSPMV_OK;
 }
 
+// `passes` passes with part j on stream j mod kStreams (see g_streams).
+int enqueue_concurrent(clspmv_gpumat_s* g, int passes, hipStream_t st) {
+    const int ns = (int)std::min<size_t>(kStreams, g->parts.size());
+    while ((int)g->aux.size() < ns - 1) {
+        hipStream_t s = nullptr;
+        hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_status(e, "hipStreamCreate");
+        g->aux.push_back(s);
+    }
+    while ((int)g->ev.size() < ns) {
+        hipEvent_t ev = nullptr;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_status(e, "hipEventCreate");
+        g->ev.push_back(ev);
+    }
+    hipError_t e = hipEventRecord(g->ev[0], st);
+    for (int a = 0; a < ns - 1 && e == hipSuccess; ++a) e = hipStreamWaitEvent(g->aux[(size_t)a], g->ev[0], 0);
+    if (e != hipSuccess) return hip_status(e, "fork");
+    const float* x = (const float*)g->x.ptr;
+    float* y = (float*)g->y.ptr;
+    for (int i = 0; i < passes; ++i)
+        for (size_t j = 0; j < g->parts.size(); ++j) {
+            const int k = (int)(j % (size_t)ns);
+            const int r = clspmv_part_launch(g->parts[j], x, y + g->row0[j], k ? g->aux[(size_t)k - 1] : st);
+            if (r) return hip_status((hipError_t)r, "do_spmv launch");
+        }
+    for (int a = 0; a < ns - 1 && e == hipSuccess; ++a) {
+        e = hipEventRecord(g->ev[(size_t)a + 1], g->aux[(size_t)a]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, g->ev[(size_t)a + 1], 0);
+    }
+    return e == hipSuccess ? CLSPMV_OK : hip_status(e, "join");
+}
+
 // ntimes passes on the context stream: whole graphs of kGraphPasses passes,
 // then one graph of the remainder.
 int run_passes(clspmv_gpumat_s* g, int ntimes, bool iterate) {
     hipStream_t st = g->ctx->stream();
+    if (!g_use_graph && !iterate && g_streams && g->disjoint && g->parts.size() > 1 && ntimes > 0)
+        return enqueue_concurrent(g, ntimes, st);
     if (!g_use_graph || g->parts.empty()) return enqueue_passes(g, ntimes, iterate, st);
     for (int done = 0; done < ntimes;) {
         const int n = std::min(kGraphPasses, ntimes - done);
@@ -266,14 +323,18 @@ constexpr int kOneRun = 1;
 // Row-run classes.  Local runs are refined by shape: rows whose entries lie
 // on at most 64 diagonals that they fill to >= 2/3 run as DIA (4 bytes per
 // slot, x read along the diagonals); rows of dense aligned b x b blocks
-// (b = 4, 3 or 2; >= 95 % of the entries in full blocks) as native sliced
-// BSR; rows of near-equal length (SELL with 32-row slices pads <= 25 %) as
-// SELL; other rows as CSR row blocks.  The classes that change the layout
-// of a single-run matrix (DIA, BSR) are the ones >= kLocalDia.
+// (b = 4 or 2; bsr_fits) as native BSR; rows of near-equal length (SELL
+// with 32-row slices pads <= 25 %) as SELL, scattered ones too while x fits
+// one XCD's L2; other rows as CSR row blocks.  "Scattered": under 3/4 of
+// the entries within min(65536, max(1024, cols / 8)) columns of the
+// diagonal.  The layout of each class by the run's size: run_layout.  The classes that
+// change the layout of a single-run matrix (DIA, BSR) are the ones
+// >= kLocalDia.
 enum RunClass : signed char {
     kScattered = 0,
     kLocalCsr = 1,
-    kLocalSell = 2,
+    kSell = 2,
+    kScatteredSell = 3,  // near-equal rows with scattered columns, x within L2
     kLocalDia = 10,
     kLocalBsr2 = 12,
     kLocalBsr3 = 13,
@@ -281,37 +342,69 @@ enum RunClass : signed char {
 };
 constexpr int kDiaMaxDiagonals = 64;
 
-// Share of a row block's entries in full b x b blocks (block rows counted
-// from r0, which is a multiple of b).
-double full_block_share(const clspmv_csr_matrix& m, const std::vector<int>& rp, int r0, int r1, int b) {
-    long long n = 0, full = 0;
+// Block statistics of rows [r0, r1) cut into b x b blocks (block rows
+// counted from r0, which is a multiple of b): entries in full blocks, all
+// entries, blocks, and the slots of the sliced BSR layout (each 64-block-row
+// slice padded to its longest block row).
+struct BlockStats {
+    long long full = 0, n = 0, blocks = 0, slots = 0;
+};
+BlockStats block_stats(const clspmv_csr_matrix& m, const std::vector<int>& rp, int r0, int r1, int b) {
+    BlockStats st;
     std::vector<int> keys;
+    long long slice_max = 0;
+    int in_slice = 0;
     for (int br = r0; br < r1; br += b) {
         keys.clear();
         for (int r = br; r < std::min(r1, br + b); ++r)
             for (int k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) keys.push_back(m.csr_col_id[k] / b);
         std::sort(keys.begin(), keys.end());
-        n += (long long)keys.size();
+        st.n += (long long)keys.size();
+        long long nb = 0;
         for (size_t i = 0; i < keys.size();) {
             size_t j = i;
             while (j < keys.size() && keys[j] == keys[i]) ++j;
-            if ((long long)(j - i) == (long long)b * b) full += (long long)(j - i);
+            if ((long long)(j - i) == (long long)b * b) st.full += (long long)(j - i);
+            ++nb;
             i = j;
         }
+        st.blocks += nb;
+        slice_max = std::max(slice_max, nb);
+        if (++in_slice == 64) {
+            st.slots += 64 * slice_max;
+            slice_max = 0;
+            in_slice = 0;
+        }
     }
-    return n ? (double)full / (double)n : 0.0;
+    if (in_slice) st.slots += in_slice * slice_max;  // a partial slice: its rows' own padding
+    return st;
+}
+
+// Share of a row block's entries in full b x b blocks.
+double full_block_share(const clspmv_csr_matrix& m, const std::vector<int>& rp, int r0, int r1, int b) {
+    const BlockStats st = block_stats(m, rp, r0, r1, b);
+    return st.n ? (double)st.full / (double)st.n : 0.0;
+}
+
+// Rows [r0, r1) as sliced BSR b x b: >= 95 % of the entries in full blocks
+// and the slices' padding at most half the blocks (a few long scattered rows
+// among block rows would pad whole slices to their length).
+bool bsr_fits(const clspmv_csr_matrix& m, const std::vector<int>& rp, int r0, int r1, int b) {
+    const BlockStats st = block_stats(m, rp, r0, r1, b);
+    return st.n && 100 * st.full >= 95 * st.n && 2 * st.slots <= 3 * st.blocks;
 }
 
 signed char classify_block(const clspmv_csr_matrix& m, const std::vector<int>& rp, int r0, int r1) {
     long long n = 0, near = 0;
+    const long long win = std::min<long long>(kLocalWindow, std::max(1024, m.matinfo.width / 8));
     for (int r = r0; r < r1; ++r)
         for (int k = rp[(size_t)r]; k < rp[(size_t)r + 1]; ++k) {
             const long long d = (long long)m.csr_col_id[k] - r;
-            near += (d <= kLocalWindow && d >= -kLocalWindow);
+            near += (d <= win && d >= -win);
             ++n;
         }
     if (n == 0) return -1;
-    if (4 * near < 3 * n) return kScattered;
+    const bool scattered = 4 * near < 3 * n;
     // repeated (row, col) entries (a matrix file may hold them) are summed
     // by the row-list layouts but would collide in a block: no BSR then
     bool repeats = false;
@@ -339,35 +432,73 @@ signed char classify_block(const clspmv_csr_matrix& m, const std::vector<int>& r
         off.erase(std::unique(off.begin(), off.end()), off.end());
         if (few && (int)off.size() <= kDiaMaxDiagonals && 3 * n >= 2LL * (long long)off.size() * (r1 - r0))
             return kLocalDia;
+        if (bsr_fits(m, rp, r0, r1, 4)) return kLocalBsr4;
+        if (bsr_fits(m, rp, r0, r1, 2)) return kLocalBsr2;
     }
-    if (!repeats && full_block_share(m, rp, r0, r1, 4) >= 0.95) return kLocalBsr4;
-    if (!repeats && full_block_share(m, rp, r0, r1, 2) >= 0.95) return kLocalBsr2;
     long long slots = 0;
     for (int s = r0; s < r1; s += 32) {
         int w = 0;
         for (int r = s; r < std::min(r1, s + 32); ++r) w = std::max(w, rp[(size_t)r + 1] - rp[(size_t)r]);
         slots += 32LL * w;
     }
-    return 4 * slots <= 5 * n ? kLocalSell : kLocalCsr;
+    const bool sell = 4 * slots <= 5 * n;
+    if (scattered) return sell && !wide_x(m.matinfo.width) ? kScatteredSell : kScattered;
+    return sell ? kSell : kLocalCsr;
 }
 
-// Method of a row run of class cls: the layout its part is built with.
-int run_method(signed char cls, int rows, int cols, long long nnz) {
-    if (cls == kScattered) return !wide_x(cols) ? 1 : xtile_pays(rows, cols, nnz) ? kXtile : 2;
-    return cls == kLocalCsr ? 1 : 0;
+// Layout (format, method) of a row run of class cls with `rows` rows and
+// n entries.  Sizes fitted on MI355X against the autotuner's timings of
+// 2400 matrices of 2^10..2^20 rows:
+//  * scattered rows: x-tiled (method 4) once the run has >= 2^20.3 entries
+//    and >= 6 per row, also with x inside L2 (Zipf rows: 1.2-1.6x faster
+//    than adaptive row blocks above that, slower below); with x wider than
+//    L2 by the cost rule (x-tiled or column-sliced); else adaptive rows;
+//  * near-equal scattered rows: SELL, x-tiled from 2^21.5 entries and >= 12
+//    per row (uniform rows: 1.1-1.8x faster there);
+//  * DIA: one row per lane from 1.5M entries, below that 2-8 lanes per row
+//    (all of a row's diagonals in one round of loads; small 27-point
+//    stencils ran ~1.5x slower one row per lane than 2 x 2 BSR lanes);
+//  * BSR: the sliced layout (one lane per block row) from 2^21 entries;
+//    below, 2-16 lanes per block row (method 1) with 2 x 2 blocks for 4 x 4
+//    ones -- more waves for a small run (4 x 4 block matrices of 2^11..2^16
+//    rows: 2 x 2 lanes 1.1-1.4x faster than 4 x 4 lanes, 1.2-1.8x than the
+//    sliced 4 x 4 kernel).
+void run_layout(signed char cls, int rows, int cols, long long n, int* fmt, int* meth) {
+    const double mean = rows ? (double)n / rows : 0.0;
+    *fmt = CLSPMV_FMT_CSR_VECTOR;
+    *meth = 1;
+    switch (cls) {
+        case kScattered:
+            if (wide_x(cols)) *meth = xtile_pays(rows, cols, n) ? kXtile : 2;
+            else if (n >= (1LL << 20) * 5 / 4 && mean >= 6.0) *meth = kXtile;
+            break;
+        case kScatteredSell:
+            if (n >= 3LL << 20 && mean >= 12.0) *meth = kXtile;
+            else *fmt = CLSPMV_FMT_SELL, *meth = 0;
+            break;
+        case kSell: *fmt = CLSPMV_FMT_SELL, *meth = 0; break;
+        case kLocalDia: *fmt = CLSPMV_FMT_DIA, *meth = n < (3LL << 19) ? 4 : 0; break;
+        case kLocalBsr2:
+        case kLocalBsr3:
+        case kLocalBsr4:
+            *fmt = CLSPMV_FMT_BSR;
+            *meth = n >= (1LL << 21) ? 0 : 1;
+            break;
+        default: break;  // local CSR rows: adaptive row blocks
+    }
 }
-int run_format(signed char cls) {
-    return cls == kLocalSell                                             ? CLSPMV_FMT_SELL
-           : cls == kLocalDia                                            ? CLSPMV_FMT_DIA
-           : (cls == kLocalBsr2 || cls == kLocalBsr3 || cls == kLocalBsr4) ? CLSPMV_FMT_BSR
-                                                                           : CLSPMV_FMT_CSR_VECTOR;
+
+// Block size of a BSR run's part: the class's, 2 for 4 x 4 blocks in lanes.
+unsigned run_block(signed char cls, int meth) {
+    const unsigned b = cls == kLocalBsr4 ? 4 : cls == kLocalBsr3 ? 3 : 2;
+    return meth == 1 && b == 4 ? 2 : b;
 }
 
 // One run as a part of its class and method; rows [0, sub.height) of `sub`.
-int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char cls, int meth, int row0) {
+int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char cls, int fmt, int meth, int row0) {
     const int cols = sub.matinfo.width, rows = sub.matinfo.height;
     const long long n = sub.matinfo.nnz;
-    if (cls == kScattered || cls == kLocalCsr) {
+    if (fmt == CLSPMV_FMT_CSR_VECTOR) {
         if (meth != kXtile || !g_run_codec) return add_flat_part(g, sub, meth, row0);
         clspmv_part_t p = nullptr;
         const int rc = clspmv_part_csr_mixed(g->ctx, &sub, g_run_codec, &p);
@@ -388,21 +519,23 @@ int add_run_part(clspmv_gpumat_s* g, const clspmv_csr_matrix& sub, signed char c
             coo.coo_data[k] = sub.csr_data[k];
         }
     clspmv_part_t p = nullptr;
-    if (cls == kLocalSell) {
+    if (fmt == CLSPMV_FMT_SELL) {
         clspmv_sell_matrix sell{};
         rc = clspmv_coo2sell(&coo, &sell, 32);
         if (!rc) rc = clspmv_part_sell(g->ctx, &sell, 0, &p);
         clspmv_free_sell(&sell);
-    } else if (cls == kLocalDia) {
+    } else if (fmt == CLSPMV_FMT_DIA) {
         clspmv_dia_matrix dia{};
         rc = clspmv_coo2dia(&coo, &dia, 32);
-        if (!rc) rc = clspmv_part_dia(g->ctx, &dia, 0, &p);
+        if (!rc) rc = clspmv_part_dia(g->ctx, &dia, meth, &p);
         clspmv_free_dia(&dia);
     } else {
-        const unsigned b = cls == kLocalBsr4 ? 4 : cls == kLocalBsr3 ? 3 : 2;
+        const unsigned b = run_block(cls, meth);
         clspmv_bsr_matrix bsr{};
         rc = clspmv_coo2bsr(&coo, &bsr, b, b);
-        if (!rc && g_run_codec) {
+        if (!rc && meth == 1) {
+            rc = clspmv_part_bsr(g->ctx, &bsr, 1, &p);
+        } else if (!rc && g_run_codec) {
             rc = clspmv_part_bsr_mixed(g->ctx, &bsr, g_run_codec, 1, &p);
             if (!rc && !codec_passes(p)) p = nullptr;
         }
@@ -445,6 +578,78 @@ struct clspmv_plan_s {
 
 namespace {
 
+// Below this many entries a matrix whose rows fall into several runs runs
+// as one part of adaptive row blocks (composites of band, block and Zipf
+// rows of 2^15..2^22 entries: 3 runs 1.1-2.1x slower than that one part,
+// from 2^22 on 0.76-0.97x).  "cocktail_small_plan" sets it (0: never).
+long long g_small_plan = 1LL << 22;
+
+// Rows per sub-block when a class boundary is placed: a multiple of the
+// 32-row SELL slice and of the block heights 2 and 4.
+constexpr int kRefineRows = 128;
+
+// Places each boundary between runs of different classes (found on run
+// blocks of rb rows) at kRefineRows granularity: within the run blocks on
+// either side, at the split that keeps the most sub-blocks of the left
+// run's class before it and of the right run's class after it.  A
+// sub-block of neither class goes to the side whose class is not a layout
+// class (DIA, BSR): a composite's block rows beside its scattered rows
+// would otherwise carry them into a BSR run.  Runs left empty are dropped,
+// neighbours of one class joined, and DIA / BSR runs checked whole again
+// (reclassified when the moved rows no longer fit).
+void refine_cuts(const clspmv_csr_matrix& m, const std::vector<int>& rp, int rb, std::vector<int>& cut,
+                 std::vector<signed char>& rc) {
+    const int R = kRefineRows;
+    for (size_t i = 1; i + 1 < cut.size(); ++i) {
+        const signed char A = rc[i - 1], B = rc[i];
+        const int lo = std::max(cut[i - 1], cut[i] - rb), hi = std::min(cut[i + 1], cut[i] + rb);
+        const int nsub = (hi - lo + R - 1) / R;  // lo and the old cut are multiples of R
+        std::vector<signed char> sc((size_t)nsub);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(omp_threads())
+        for (int j = 0; j < nsub; ++j) sc[(size_t)j] = classify_block(m, rp, lo + j * R, std::min(hi, lo + (j + 1) * R));
+        // score(j): sub-blocks of class A in [0, j) + of class B in [j, nsub)
+        int right = 0;
+        for (int j = 0; j < nsub; ++j) right += sc[(size_t)j] == B;
+        const int j0 = (cut[i] - lo) / R;
+        int best = -1, bj = j0, left = 0;
+        for (int j = 0; j <= nsub; ++j) {
+            const int score = left + right;
+            bool take = score > best;
+            if (score == best) {
+                if (B >= kLocalDia && A < kLocalDia) take = true;                            // mixed rows stay out of B
+                else if (!(A >= kLocalDia && B < kLocalDia)) take = std::abs(j - j0) < std::abs(bj - j0);  // nearest
+            }
+            if (take) {
+                best = score;
+                bj = j;
+            }
+            if (j < nsub) {
+                left += sc[(size_t)j] == A;
+                right -= sc[(size_t)j] == B;
+            }
+        }
+        cut[i] = std::min(hi, lo + bj * R);
+    }
+    std::vector<int> c2{0};
+    std::vector<signed char> r2;
+    for (size_t i = 0; i + 1 < cut.size(); ++i) {
+        if (cut[i + 1] <= cut[i]) continue;
+        signed char k = rc[i];
+        if (k >= kLocalDia) {
+            const signed char again = classify_block(m, rp, cut[i], cut[i + 1]);
+            if (again != k) k = again < 0 ? kLocalCsr : again;
+        }
+        if (!r2.empty() && r2.back() == k) {
+            c2.back() = cut[i + 1];
+            continue;
+        }
+        r2.push_back(k);
+        c2.push_back(cut[i + 1]);
+    }
+    cut.swap(c2);
+    rc.swap(r2);
+}
+
 // Row runs of the merged flat rows (rp, ci, cv of the plan).  Returns
 // kOneRun (nothing planned) when single_ok is false and the rows form a
 // single run of a class that keeps the partition's own flat parts.
@@ -460,11 +665,13 @@ int plan_runs(clspmv_plan_s* P, bool single_ok) {
     const std::vector<int>& rp = P->rp;
     auto push_run = [&](signed char cls, int r0, int r1) {
         const long long n = rp[(size_t)r1] - rp[(size_t)r0];
-        P->parts.push_back(PlanPart{9, cls, run_format(cls), run_method(cls, r1 - r0, cols, n), r0, r1, n});
+        PlanPart q{9, cls, 0, 0, r0, r1, n};
+        run_layout(cls, r1 - r0, cols, n, &q.format, &q.method);
+        P->parts.push_back(q);
     };
-    // run blocks: 32768 rows, finer (down to 4096) below 512K rows so that a
-    // small matrix still has ~16 blocks to tell its regions apart
-    int rb = 4096;
+    // run blocks: 32768 rows, finer (down to 1024) below 1M rows so that a
+    // small matrix still has ~32 blocks to tell its regions apart
+    int rb = 1024;
     while (rb < kRunBlock && rb * 32 <= rows) rb *= 2;
     const int nbk = (rows + rb - 1) / rb;
     if (!g_split_local || nbk == 0) {
@@ -479,7 +686,7 @@ int plan_runs(clspmv_plan_s* P, bool single_ok) {
         const int r1 = std::min(rows, rb);
         const signed char c0 = classify_block(merged, rp, 0, r1);
         if (c0 > kScattered && c0 < kLocalDia && full_block_share(merged, rp, 0, r1, 3) >= 0.95 &&
-            full_block_share(merged, rp, 0, rows, 3) >= 0.95) {
+            bsr_fits(merged, rp, 0, rows, 3)) {
             push_run(kLocalBsr3, 0, rows);
             return CLSPMV_OK;
         }
@@ -491,15 +698,42 @@ int plan_runs(clspmv_plan_s* P, bool single_ok) {
     for (int b = 0; b < nbk; ++b)  // empty blocks join the previous run
         if (cls[(size_t)b] < 0) cls[(size_t)b] = b ? cls[(size_t)b - 1] : kLocalCsr;
     std::vector<int> cut{0};
+    std::vector<signed char> rc{cls[0]};  // class of run i = rows [cut[i], cut[i + 1])
     for (int b = 1; b < nbk; ++b)
-        if (cls[(size_t)b] != cls[(size_t)b - 1]) cut.push_back(b * rb);
+        if (cls[(size_t)b] != cls[(size_t)b - 1]) {
+            cut.push_back(b * rb);
+            rc.push_back(cls[(size_t)b]);
+        }
     cut.push_back(rows);
-    if (cut.size() == 2 && cls[0] == kLocalSell && !single_ok) {
+    if (cut.size() == 2 && cls[0] == kSell && !single_ok) {
         // one run of near-equal rows: the partition's own flat part (its
         // ELL or SELL serves such rows as well as a SELL run)
         return kOneRun;
     }
-    for (size_t i = 0; i + 1 < cut.size(); ++i) push_run(cls[(size_t)(cut[i] / rb)], cut[i], cut[i + 1]);
+    refine_cuts(merged, rp, rb, cut, rc);
+    if (cut.size() > 2 && merged.matinfo.nnz < g_small_plan) {
+        // a small matrix runs as one part of adaptive row blocks: its runs'
+        // launches, even side by side, cost more than they save
+        push_run(kLocalCsr, 0, rows);
+        return CLSPMV_OK;
+    }
+    const size_t first = P->parts.size();
+    for (size_t i = 0; i + 1 < cut.size(); ++i) {
+        // neighbouring runs whose classes give one layout (scattered and
+        // local CSR rows both as adaptive row blocks, say) run as one part,
+        // its layout chosen again for the joined size
+        const PlanPart* b = P->parts.size() > first ? &P->parts.back() : nullptr;
+        int f = 0, m = 0;
+        run_layout(rc[i], cut[i + 1] - cut[i], cols, rp[(size_t)cut[i + 1]] - rp[(size_t)cut[i]], &f, &m);
+        if (b && b->format == f && b->method == m && (f != CLSPMV_FMT_BSR || run_block(b->cls, m) == run_block(rc[i], m))) {
+            const signed char k = b->cls == kScattered || rc[i] == kScattered ? kScattered : b->cls;
+            const int r0 = b->row0;
+            P->parts.pop_back();
+            push_run(k, r0, cut[i + 1]);
+            continue;
+        }
+        push_run(rc[i], cut[i], cut[i + 1]);
+    }
     return CLSPMV_OK;
 }
 
@@ -623,7 +857,7 @@ int upload_plan(clspmv_gpumat_s* g, const clspmv_plan_s* P) {
             sub.csr_row_ptr = sub_rp.data();
             sub.csr_col_id = const_cast<int*>(P->ci.data()) + P->rp[(size_t)q.row0];
             sub.csr_data = const_cast<float*>(P->cv.data()) + P->rp[(size_t)q.row0];
-            rc = add_run_part(g, sub, q.cls, q.method, q.row0);
+            rc = add_run_part(g, sub, q.cls, q.format, q.method, q.row0);
         }
         if (rc) return rc;
     }
@@ -654,6 +888,10 @@ int init_from_plan(clspmv_ctx_t ctx, const clspmv_plan_s* P, const float* vec, c
     if (!rc && vec) rc = clspmv_memcpy_h2d(ctx, g->x.ptr, vec, (size_t)g->width * 4);
     if (!rc && res) rc = clspmv_memcpy_h2d(ctx, g->y.ptr, res, (size_t)g->height * 4);
     if (!rc) rc = upload_plan(g, P);
+    if (!rc) {
+        g->disjoint = !P->parts.empty();
+        for (const PlanPart& q : P->parts) g->disjoint = g->disjoint && q.source == 9;
+    }
     if (!rc) rc = clspmv_synchronize(ctx);
     if (rc) {
         delete g;
@@ -873,6 +1111,15 @@ __attribute__((visibility("hidden"))) int clspmv_cocktail_option(const char* nam
     }
     if (name && std::strcmp(name, "cocktail_graph") == 0) {
         g_use_graph = value != 0;
+        return CLSPMV_OK;
+    }
+    if (name && std::strcmp(name, "cocktail_small_plan") == 0) {
+        if (value < 0) return CLSPMV_E_INVALID;
+        g_small_plan = value;
+        return CLSPMV_OK;
+    }
+    if (name && std::strcmp(name, "cocktail_streams") == 0) {
+        g_streams = value != 0;
         return CLSPMV_OK;
     }
     if (name && std::strcmp(name, "cocktail_split_local") == 0) {

@@ -10,9 +10,9 @@
 //      STREAM  rows [r0,r1) with <= 512 nnz in total and no row longer than
 //              64: the wave streams the block's nnz range into LDS with
 //              fully coalesced loads (8 per lane), then lane j sums row j;
-//      VECTOR  one row of 65..8192 nnz: the wave strides the row and
+//      VECTOR  one row of 65..1024 nnz: the wave strides the row and
 //              reduces across its 64 lanes;
-//      LONG    one 8192-nnz chunk of a longer row: the wave's partial is
+//      LONG    one 1024-nnz chunk of a longer row: the wave's partial is
 //              stored in FP64 and a second kernel adds the partials of each
 //              long row in order (deterministic, no atomics).
 #include "kcommon.hpp"

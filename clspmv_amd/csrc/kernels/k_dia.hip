@@ -5,8 +5,11 @@
 // read with a bounds test instead of the reference's zero-padded copy
 // (vec_extended, A/src/spmv_cocktail.cpp:253-301).
 //   DIA  method 0: one row per lane (A/kernels/spmv_dia.cl:4-28 "gpu_dia");
-//        method 1: four rows per lane, dwordx4 data and x loads, a
-//                  branch-free loop for interior workgroups ("gpu_dia_v4").
+//        method 1: four rows per lane, dwordx4 data and x loads (x at
+//                  4-byte alignment), a branch-free loop for interior
+//                  workgroups ("gpu_dia_v4");
+//        method 4: 2-8 lanes per row splitting the diagonals (small
+//                  matrices; no reference counterpart).
 //   BDIA method 0: one row per lane, diagonals of a band read x directly
 //                  (A/kernels/spmv_bdia.cl:46-79 "gpu_bdia_nlvec");
 //        method 1: the x window of each band (256 + width - 1 values) is
@@ -32,19 +35,63 @@ __global__ __launch_bounds__(256) void dia_row_kernel(const int* __restrict__ of
     if (row >= rows) return;
     float acc = y[row];
     const float* dp = data + row;
-    int d = 0;
-    for (; d + 4 <= ndiag; d += 4) {
-        float v[4], xv[4];
+    // 8 diagonals' loads in flight per step, the last step's missing ones
+    // skipped by a wave-uniform test: 27 diagonals take 4 dependent steps
+    // (6 + 3 serial ones with steps of 4), which is what bounds a small
+    // matrix; the sums keep diagonal order
+    for (int d = 0; d < ndiag; d += 8) {
+        float v[8], xv[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            v[u] = ld_stream(dp + (size_t)(d + u) * L);
-            xv[u] = xload(x, row + offsets[d + u], cols);
+        for (int u = 0; u < 8; ++u) {
+            v[u] = 0.f;
+            xv[u] = 0.f;
+            if (d + u < ndiag) {
+                v[u] = ld_stream(dp + (size_t)(d + u) * L);
+                xv[u] = xload(x, row + offsets[d + u], cols);
+            }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc = fmaf(v[u], xv[u], acc);
+        for (int u = 0; u < 8; ++u)
+            if (d + u < ndiag) acc = fmaf(v[u], xv[u], acc);
     }
-    for (; d < ndiag; ++d) acc = fmaf(ld_stream(dp + (size_t)d * L), xload(x, row + offsets[d], cols), acc);
     y[row] = acc;
+}
+
+// G lanes per row (method 4, for small matrices): lane t of a row's group
+// takes diagonals t, t + G, ..., at most 8 of them, so all of a row's loads
+// issue in one round where the one-row-per-lane kernel takes ndiag / 8
+// dependent rounds -- the bound of a matrix too small to fill the GPU (a
+// 27-point stencil of 10^3..30^3 rows: ~4 us one row per lane, the launch
+// floor ~2.6 us).  The G partial sums are added by log2 G lane exchanges
+// (fixed order: deterministic).
+template <int G>
+__global__ __launch_bounds__(256) void dia_lanes_kernel(const int* __restrict__ offsets,
+                                                        const float* __restrict__ data, int L, int ndiag,
+                                                        const float* __restrict__ x, float* __restrict__ y, int rows,
+                                                        int cols) {
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int row = gid / G, t = gid % G;
+    float acc = 0.f;
+    if (row < rows) {
+        for (int d0 = t; d0 < ndiag; d0 += 8 * G) {
+            float v[8], xv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int d = d0 + u * G;
+                v[u] = 0.f;
+                xv[u] = 0.f;
+                if (d < ndiag) {
+                    v[u] = ld_stream(data + (size_t)d * L + row);
+                    xv[u] = xload(x, row + offsets[d], cols);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = fmaf(v[u], xv[u], acc);
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) acc += __shfl_xor(acc, m);
+    if (row < rows && t == 0) y[row] += acc;
 }
 
 // Four consecutive rows per lane: one dwordx4 value load and one dwordx4
@@ -72,9 +119,8 @@ __global__ __launch_bounds__(256) void dia_row4_kernel(const int* __restrict__ o
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 v[u] = ld_stream(data + (size_t)(d + u) * L4 + r4);
-                const float* xp = x + row + offsets[d + u];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) xv[u][j] = xp[j];
+                const f32x4a4 t = *reinterpret_cast<const f32x4a4*>(x + row + offsets[d + u]);
+                xv[u][0] = t.x, xv[u][1] = t.y, xv[u][2] = t.z, xv[u][3] = t.w;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -86,11 +132,11 @@ __global__ __launch_bounds__(256) void dia_row4_kernel(const int* __restrict__ o
         }
         for (; d < ndiag; ++d) {
             const float4 v = ld_stream(data + (size_t)d * L4 + r4);
-            const float* xp = x + row + offsets[d];
-            a0 = fmaf(v.x, xp[0], a0);
-            a1 = fmaf(v.y, xp[1], a1);
-            a2 = fmaf(v.z, xp[2], a2);
-            a3 = fmaf(v.w, xp[3], a3);
+            const f32x4a4 t = *reinterpret_cast<const f32x4a4*>(x + row + offsets[d]);
+            a0 = fmaf(v.x, t.x, a0);
+            a1 = fmaf(v.y, t.y, a1);
+            a2 = fmaf(v.z, t.z, a2);
+            a3 = fmaf(v.w, t.w, a3);
         }
     } else {
         for (int d = 0; d < ndiag; ++d) {
@@ -165,7 +211,19 @@ __global__ __launch_bounds__(256) void bdia_lds_kernel(const int* __restrict__ b
 
 int launch_dia(const DevDia& m, int method, const float* x, float* y, hipStream_t st) {
     if (m.rows == 0) return 0;
-    if (method == 1 && (m.L % 4) == 0) {
+    if (method == 4) {
+        const long long threads = (long long)m.rows * (m.ndiag > 32 ? 8 : m.ndiag > 16 ? 4 : 2);
+        const dim3 grid((unsigned)((threads + 255) / 256));
+        if (m.ndiag > 32)
+            hipLaunchKernelGGL(dev::dia_lanes_kernel<8>, grid, dim3(256), 0, st, m.offsets, m.data, m.L, m.ndiag, x, y,
+                               m.rows, m.cols);
+        else if (m.ndiag > 16)
+            hipLaunchKernelGGL(dev::dia_lanes_kernel<4>, grid, dim3(256), 0, st, m.offsets, m.data, m.L, m.ndiag, x, y,
+                               m.rows, m.cols);
+        else
+            hipLaunchKernelGGL(dev::dia_lanes_kernel<2>, grid, dim3(256), 0, st, m.offsets, m.data, m.L, m.ndiag, x, y,
+                               m.rows, m.cols);
+    } else if (method == 1 && (m.L % 4) == 0) {
         int r4 = (m.rows + 3) / 4;
         hipLaunchKernelGGL(dev::dia_row4_kernel, dim3((r4 + 255) / 256), dim3(256), 0, st, m.offsets,
                            reinterpret_cast<const float4*>(m.data), m.L / 4, m.ndiag, m.offmin, m.offmax, x, y,

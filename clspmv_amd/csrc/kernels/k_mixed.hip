@@ -40,10 +40,14 @@ __device__ __forceinline__ float xload_m(const float* __restrict__ x, int c, int
     return (c >= 0 && c < cols) ? x[c] : 0.f;
 }
 
-// R consecutive floats at p (p aligned to 4 R bytes) as one load / store.
+// R consecutive floats at p (p aligned to 4 R bytes, 16 for R = 8) as one
+// load / store (two for R = 8).
 template <int R>
 __device__ __forceinline__ void load_vec(const float* p, float (&v)[R]) {
-    if constexpr (R == 4) {
+    if constexpr (R == 8) {
+        const float4 t = *reinterpret_cast<const float4*>(p), u = *reinterpret_cast<const float4*>(p + 4);
+        v[0] = t.x, v[1] = t.y, v[2] = t.z, v[3] = t.w, v[4] = u.x, v[5] = u.y, v[6] = u.z, v[7] = u.w;
+    } else if constexpr (R == 4) {
         const float4 t = *reinterpret_cast<const float4*>(p);
         v[0] = t.x, v[1] = t.y, v[2] = t.z, v[3] = t.w;
     } else if constexpr (R == 2) {
@@ -55,43 +59,77 @@ __device__ __forceinline__ void load_vec(const float* p, float (&v)[R]) {
 }
 template <int R>
 __device__ __forceinline__ void store_vec(float* p, const float (&v)[R]) {
-    if constexpr (R == 4)
+    if constexpr (R == 8) {
         *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
-    else if constexpr (R == 2)
+        *reinterpret_cast<float4*>(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+    } else if constexpr (R == 4) {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 2) {
         *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
-    else
+    } else {
         p[0] = v[0];
+    }
 }
 
-// Rows per lane of the DIA kernel: every lane loads one 32-bit word of
-// values per diagonal (1 BF16 hi+lo word, 2 FP16 or 4 FP8 values).
-template <int CODEC>
-constexpr int dia_rows_per_lane() {
-    return CODEC == kCodecFp8Slice ? 4 : codec_row_scaled(CODEC) ? 2 : 1;
-}
-
-// The R values of rows row .. row+R-1 in one word, decoded; FP16 and int16
-// values are returned unscaled (the row scale is applied to the row sums),
-// FP8 values times the slice scale s.
-template <int CODEC, int R>
-__device__ __forceinline__ void dia_decode(unsigned w, float s, float (&v)[R]) {
-    if constexpr (CODEC == kCodecBf16x2) {
-        v[0] = bf16x2_value(w);
-    } else if constexpr (CODEC == kCodecFp16Row) {
-        v[0] = f16_value((uint16_t)(w & 0xffffu));
-        v[1] = f16_value((uint16_t)(w >> 16));
-    } else if constexpr (CODEC == kCodecI16Row) {
-        v[0] = i16_lo(w);
-        v[1] = i16_hi(w);
+// x[c .. c+R-1] (all inside x): 16-byte loads at 4-byte alignment, one per
+// four values (a wave then reads each diagonal's x run with R / 4
+// instructions instead of R), plain loads for R < 4.
+template <int R>
+__device__ __forceinline__ void load_x_run(const float* __restrict__ x, int c, float (&v)[R]) {
+    if constexpr (R % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < R / 4; ++q) {
+            const f32x4a4 t = *reinterpret_cast<const f32x4a4*>(x + c + 4 * q);
+            v[4 * q] = t.x, v[4 * q + 1] = t.y, v[4 * q + 2] = t.z, v[4 * q + 3] = t.w;
+        }
     } else {
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = s * e4m3_value((w >> (8 * j)) & 0xffu);
+        for (int j = 0; j < R; ++j) v[j] = x[c + j];
+    }
+}
+
+// Rows per lane of the DIA kernel and 32-bit words of values it loads per
+// diagonal: 1 word for BF16 hi+lo and for 4 FP8 values, 2 words (one
+// dwordx2) for 4 FP16 or int16 values.  More rows per lane means fewer
+// waves: at 8 rows per lane a 128^3 stencil had 4 waves per SIMD and ran
+// slower than at 2.
+template <int CODEC>
+constexpr int dia_rows_per_lane() {
+    return CODEC == kCodecFp8Slice ? 4 : codec_row_scaled(CODEC) ? 4 : 1;
+}
+template <int CODEC>
+constexpr int dia_words_per_lane() {
+    return codec_row_scaled(CODEC) ? 2 : 1;
+}
+
+// The R values of rows row .. row+R-1 in W words, decoded; FP16 and int16
+// values are returned unscaled (the row scale is applied to the row sums),
+// FP8 values times the slice scale s.
+template <int CODEC, int R, int W>
+__device__ __forceinline__ void dia_decode(const unsigned (&w)[W], float s, float (&v)[R]) {
+    if constexpr (CODEC == kCodecBf16x2) {
+        v[0] = bf16x2_value(w[0]);
+    } else if constexpr (CODEC == kCodecFp16Row) {
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            v[2 * i] = f16_value((uint16_t)(w[i] & 0xffffu));
+            v[2 * i + 1] = f16_value((uint16_t)(w[i] >> 16));
+        }
+    } else if constexpr (CODEC == kCodecI16Row) {
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            v[2 * i] = i16_lo(w[i]);
+            v[2 * i + 1] = i16_hi(w[i]);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = s * e4m3_value((w[0] >> (8 * j)) & 0xffu);
     }
 }
 
 // R = dia_rows_per_lane consecutive rows per lane (a wave's value loads per
-// diagonal: 256 contiguous bytes for every codec) and R consecutive x
-// values per diagonal (one multi-dword load).  A workgroup covers 256 R
+// diagonal: 256 contiguous bytes, 1 KiB for FP16 / int16) and R consecutive
+// x values per diagonal.  A workgroup covers 256 R
 // rows; when every column they can touch (its rows plus the smallest and
 // largest offset) lies inside x -- all but the first and last workgroups of
 // a banded matrix -- it takes the branch-free interior loop, chosen once per
@@ -111,10 +149,23 @@ __global__ __launch_bounds__(256) void dia_mixed_kernel(const int* __restrict__ 
     const unsigned* words = reinterpret_cast<const unsigned*>(data);
     const int sl = __builtin_amdgcn_readfirstlane(row >> 8);
     auto slice_scale = [&](int d) { return CODEC == kCodecFp8Slice ? scale[(size_t)d * nsl + sl] : 1.f; };
-    auto word = [&](int d) {
-        const size_t i = ((size_t)d * L + row) / R;
-        if constexpr (CODEC == kCodecBf16x2) return ld_stream(words + i);
-        return words[i];
+    constexpr int W = dia_words_per_lane<CODEC>();
+    // words of diagonal d for this lane's rows: a word holds R / W values, so
+    // value d L + row is word (d L + row) / R * W (L and row are multiples of
+    // R; the W words are 4 W bytes aligned)
+    auto word = [&](int d, unsigned (&w)[W]) {
+        const size_t i = ((size_t)d * L + row) / R * W;
+        if constexpr (W == 4) {
+            const uint4 t = *reinterpret_cast<const uint4*>(words + i);
+            w[0] = t.x, w[1] = t.y, w[2] = t.z, w[3] = t.w;
+        } else if constexpr (W == 2) {
+            const uint2 t = *reinterpret_cast<const uint2*>(words + i);
+            w[0] = t.x, w[1] = t.y;
+        } else if constexpr (CODEC == kCodecBf16x2) {
+            w[0] = ld_stream(words + i);
+        } else {
+            w[0] = words[i];
+        }
     };
     // y is read before the diagonals (its latency overlaps theirs), R values
     // at once unless the lane holds the end of y
@@ -130,36 +181,34 @@ __global__ __launch_bounds__(256) void dia_mixed_kernel(const int* __restrict__ 
 #pragma unroll
     for (int j = 0; j < R; ++j) a[j] = 0.f;
     if (interior) {
-        int d = 0;
-        for (; d + 4 <= ndiag; d += 4) {
-            unsigned w[4];
-            float xv[4][R];
+        // U diagonals' loads in flight per step, the last step's missing ones
+        // skipped by a wave-uniform test (no serial remainder loop); the
+        // sums keep diagonal order
+        constexpr int U = R >= 4 ? 6 : 8;
+        for (int d = 0; d < ndiag; d += U) {
+            unsigned w[U][W];
+            float xv[U][R];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                w[u] = word(d + u);
-                const float* xp = x + row + offsets[d + u];
+            for (int u = 0; u < U; ++u)
+                if (d + u < ndiag) {
+                    word(d + u, w[u]);
+                    load_x_run<R>(x, row + offsets[d + u], xv[u]);
+                }
 #pragma unroll
-                for (int j = 0; j < R; ++j) xv[u][j] = xp[j];
-            }
+            for (int u = 0; u < U; ++u)
+                if (d + u < ndiag) {
+                    float v[R];
+                    dia_decode<CODEC, R, W>(w[u], slice_scale(d + u), v);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                float v[R];
-                dia_decode<CODEC, R>(w[u], slice_scale(d + u), v);
-#pragma unroll
-                for (int j = 0; j < R; ++j) a[j] = fmaf(v[j], xv[u][j], a[j]);
-            }
-        }
-        for (; d < ndiag; ++d) {
-            float v[R];
-            dia_decode<CODEC, R>(word(d), slice_scale(d), v);
-            const float* xp = x + row + offsets[d];
-#pragma unroll
-            for (int j = 0; j < R; ++j) a[j] = fmaf(v[j], xp[j], a[j]);
+                    for (int j = 0; j < R; ++j) a[j] = fmaf(v[j], xv[u][j], a[j]);
+                }
         }
     } else {
         for (int d = 0; d < ndiag; ++d) {
+            unsigned w[W];
+            word(d, w);
             float v[R];
-            dia_decode<CODEC, R>(word(d), slice_scale(d), v);
+            dia_decode<CODEC, R, W>(w, slice_scale(d), v);
             const int c = row + offsets[d];
 #pragma unroll
             for (int j = 0; j < R; ++j) a[j] = fmaf(v[j], xload_m(x, c + j, cols), a[j]);

@@ -212,7 +212,10 @@ int launch_empty(hipStream_t st, int groups = 1, int threads = 64);
 constexpr int kCsrStreamPerLane = 8;                    // nnz per lane in a stream block
 constexpr int kCsrStreamMaxNnz = 64 * kCsrStreamPerLane;  // 512
 constexpr int kCsrStreamMaxRowLen = 64;                 // longer rows get their own wave
-constexpr int kCsrLongChunk = 8192;                     // nnz per wave for very long rows
+// nnz per wave for long rows: a wave takes 256 entries per step, so a chunk
+// is 4 dependent gather steps; 8192-entry chunks made one wave's latency
+// chain (~40 us) the kernel's tail on Zipf rows of a small matrix
+constexpr int kCsrLongChunk = 1024;
 constexpr int kCooPerLane = 8;
 constexpr int kCooChunk = 64 * kCooPerLane;             // 512 nnz per wave
 

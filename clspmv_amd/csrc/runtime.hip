@@ -220,7 +220,10 @@ const char* codec_name(int codec) {
 }
 
 // LDS-windowed sliced BSR kernels (tunable: "bsr_lds_window", 0 / 1).
-bool g_bsr_window = true;
+// 0: never; 1 (default): by the rules of bsr_windows; 2: whenever every
+// window fits LDS (no size or width rule; for tests on small matrices).
+int g_bsr_window = 1;
+constexpr int kBsrWinMinGroups = 128;
 
 // Column repeated by the padding slots of an empty block row of the sliced
 // BSR layout: the first column of the slice's first nonempty block row
@@ -244,7 +247,9 @@ std::vector<int> bsr_slice_fill(const clspmv_bsr_matrix* m, int brows) {
 // arithmetic, window start + 4 kBsrWinFloats, could overflow), some window
 // holds more than kBsrWinFloats, or
 // the windows are narrow: a mean under 2 kBsrWinRows bw columns (twice the
-// rows' own width).  There the plain kernel's gathers hit few cache lines
+// rows' own width), or there are fewer than kBsrWinMinGroups windows (one
+// 1024-lane workgroup per window: 4096 block rows would run on 4 of the
+// 256 CUs).  There the plain kernel's gathers hit few cache lines
 // per wave and its higher occupancy wins (1M 3x3 block rows, 9 blocks each
 // within +-b block columns: plain 71-74 us for b <= 200, 85 us at b = 1000,
 // 98 us at b = 5000; windowed 81 us throughout).
@@ -253,6 +258,8 @@ std::vector<int2> bsr_windows(const std::vector<int>& sp, const std::vector<int>
     if (!g_bsr_window || brows == 0 || cols < 4 || cols > INT32_MAX - 4 * kBsrWinFloats) return {};
     const int ns = (int)sp.size() - 1, per = kBsrWinRows / kBsrSlice;
     const int nwg = (brows + kBsrWinRows - 1) / kBsrWinRows;
+    const bool rules = g_bsr_window == 1;
+    if (rules && nwg < kBsrWinMinGroups) return {};
     std::vector<int2> w((size_t)nwg);
     long long total = 0;
     for (int g = 0; g < nwg; ++g) {
@@ -271,7 +278,7 @@ std::vector<int2> bsr_windows(const std::vector<int>& sp, const std::vector<int>
         w[(size_t)g] = make_int2(lo, (int)len);
         total += len;
     }
-    if (total < 2LL * kBsrWinRows * bw * nwg) return {};
+    if (rules && total < 2LL * kBsrWinRows * bw * nwg) return {};
     return w;
 }
 
@@ -1286,9 +1293,14 @@ int clspmv_part_dia(clspmv_ctx_t ctx, const clspmv_dia_matrix* m, int method, cl
         p->dia.cols = cols;
         p->dia.offmin = D ? *std::min_element(m->dia_offsets, m->dia_offsets + D) : 0;
         p->dia.offmax = D ? *std::max_element(m->dia_offsets, m->dia_offsets + D) : 0;
-        int meth = (method == 1 || method == 3) ? 1 : 0;
-        if (L % 4) meth = 0;
-        fill_info(p, CLSPMV_FMT_DIA, meth, rows, cols, m->matinfo.nnz, meth ? "dia_row4" : "dia_row", 1);
+        // reference methods 0 / 2 -> one row per lane, 1 / 3 -> four rows
+        // per lane; 4: 2-8 lanes per row (one row per lane at <= 8 diagonals)
+        int meth = (method == 1 || method == 3) ? 1 : (method == 4 && D > 8) ? 4 : 0;
+        if (meth == 1 && L % 4) meth = 0;
+        char name[32];
+        if (meth == 4) std::snprintf(name, sizeof(name), "dia_lanes%d", D > 32 ? 8 : D > 16 ? 4 : 2);
+        else std::snprintf(name, sizeof(name), "%s", meth ? "dia_row4" : "dia_row");
+        fill_info(p, CLSPMV_FMT_DIA, meth, rows, cols, m->matinfo.nnz, name, 1);
         return CLSPMV_OK;
     });
 }
@@ -1644,8 +1656,8 @@ int clspmv_set_option(const char* name, long long value) {
         return CLSPMV_OK;
     }
     if (std::strcmp(name, "bsr_lds_window") == 0) {
-        if (value < 0 || value > 1) return CLSPMV_E_INVALID;
-        g_bsr_window = value != 0;
+        if (value < 0 || value > 2) return CLSPMV_E_INVALID;
+        g_bsr_window = (int)value;
         return CLSPMV_OK;
     }
     if (std::strcmp(name, "xtile_blocks") == 0) {

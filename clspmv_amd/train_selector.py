@@ -1,7 +1,7 @@
 """Trains the learned format selector on autotuner-timed matrices and
 reports its regret against exhaustive autotuning on held-out matrices.
 
-    python -m clspmv_amd.train_selector --n 4000 --out DIR
+    python -m clspmv_amd.train_selector --n 2400 --log2-min 10 --log2-max 20 --out DIR
 
 DIR receives dataset.npz (features, candidate times, kinds), model.npz (the
 trained network and its feature normalisation) and report.json: per split the
@@ -20,6 +20,7 @@ import time
 
 import numpy as np
 
+from . import _lib as L
 from . import cocktail as CK
 from . import matrix as M
 from . import selector as S
@@ -54,6 +55,7 @@ def cocktail_times(ctx, idx, seed, log2_min, log2_max, detail=None, target_ms=1.
             gpu.do_spmv(2)
             ms = ctx.time_end() / 2
             n = int(min(50, max(3, target_ms / max(ms, 1e-4))))
+            gpu.do_spmv(n)  # untimed, as the autotuner does for each candidate
             ctx.time_begin()
             gpu.do_spmv(n)
             out[i] = ctx.time_end() / n
@@ -96,16 +98,21 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--eval-cocktail", metavar="DIR", default=None,
                     help="only re-time the Cocktail plan on the held-out subset of the dataset in DIR")
-    ap.add_argument("--n", type=int, default=4000)
+    ap.add_argument("--n", type=int, default=2400)
     ap.add_argument("--test-frac", type=float, default=0.2)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--log2-min", type=float, default=12)
-    ap.add_argument("--log2-max", type=float, default=18)
+    ap.add_argument("--log2-min", type=float, default=10)
+    ap.add_argument("--log2-max", type=float, default=20)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--cocktail-subset", type=int, default=200)
     ap.add_argument("--out", default=None)
-    ap.add_argument("--detail", default=None, help="with --eval-cocktail: write per-matrix records to this JSON file")
+    ap.add_argument("--detail", default=None, help="write the Cocktail plan's per-matrix records (kernels, time, best candidate) to this JSON file")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="clspmv_set_option before timing (e.g. cocktail_streams=0); repeatable")
     args = ap.parse_args(argv)
+    for opt in args.option:
+        name, value = opt.split("=", 1)
+        L.check(L.lib().clspmv_set_option(name.encode(), int(value)), "set_option")
     ctx = M.Context(0)
     if args.eval_cocktail:
         detail = [] if args.detail else None
@@ -148,9 +155,17 @@ def main(argv=None):
         }
     sub = test[:args.cocktail_subset]
     if len(sub):
-        ck = cocktail_times(ctx, sub, args.seed, args.log2_min, args.log2_max)
-        r = [ck[int(i)] / times[i].min() - 1.0 for i in sub]
+        rec = {}
+        ck = cocktail_times(ctx, sub, args.seed, args.log2_min, args.log2_max, rec)
+        r = np.array([ck[int(i)] / times[i].min() - 1.0 for i in sub])
         report["test"]["cocktail_mi355x_tables"] = summary(r)
+        report["test"]["cocktail_mi355x_tables_by_kind"] = {
+            k: summary(r[kinds[sub] == k]) for k in sorted(set(kinds[sub].tolist()))}
+        if args.detail:
+            with open(args.detail, "w") as f:
+                json.dump([dict(rec[int(i)], index=int(i), best=S.CANDIDATE_NAMES[int(np.argmin(times[i]))],
+                                best_ms=float(times[i].min()), regret=float(ck[int(i)] / times[i].min() - 1.0))
+                           for i in sub], f, indent=0)
         pick = sel.predict(feats[sub], times[sub])
         report["test"]["network_on_cocktail_subset"] = summary(S.regret(times[sub], pick))
     report["seconds"] = round(time.perf_counter() - t0, 1)

@@ -323,7 +323,10 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
  *   CSR vec  0 L lanes per row, 1 adaptive row blocks, 2 column-sliced,
  *            4 LDS x-tiled
  *   COO      0 segmented reduction, 2 column-sliced, 4 LDS x-tiled
- *   DIA      0 one row per lane, 1 four rows per lane
+ *   DIA      0 one row per lane, 1 four rows per lane (reference methods
+ *            2 / 3 map to 0 / 1), 4 two to eight lanes per row splitting
+ *            the diagonals -- all of a row's loads in one round, for small
+ *            matrices (one row per lane at <= 8 diagonals)
  *   BDIA     0 direct x reads, 1 LDS-staged x band
  *   BSR      0 sliced (64 block rows per slice; for r x r blocks with
  *            2 <= r <= 4, block columns stored as int16 steps within a
@@ -371,8 +374,9 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  *                       layouts (default 2 MiB);
  *   "bsr_lds_window"    1 (default): sliced BSR parts (FP32 and mixed) stage
  *                       each 1024-block-row workgroup's x window in LDS
- *                       when every window fits (<= 40960 columns) and their
- *                       mean exceeds 2048 bw columns; 0: never;
+ *                       when every window fits (<= 40960 columns), their
+ *                       mean exceeds 2048 bw columns and there are >= 128
+ *                       windows; 2: whenever every window fits; 0: never;
  *   "xtile_blocks"      target workgroups of the x-tiled layout (default
  *                       256, one per CU);
  *   "xtile_max_groups"  column groups per row block of the x-tiled layout
@@ -382,10 +386,10 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  *                       flat parts (ELL, SELL, CSR, COO) of a partition as
  *                       one merged row-list part; 0: one part each;
  *   "cocktail_split_local" 1 (default): the merged flat rows are cut into
- *                       runs of 32768-row blocks whose columns lie near the
- *                       diagonal (CSR row blocks) or not (x-tiled or
- *                       column-sliced layout), one part per run; 0: one
- *                       part;
+ *                       row runs by class (scattered, CSR row blocks, SELL,
+ *                       DIA, sliced BSR; classified on blocks of 1024-32768
+ *                       rows, boundaries placed to 128 rows), one part per
+ *                       run; 0: one part;
  *   "cocktail_value_codec" 0 (default) or a CLSPMV_VAL_* codec for the row-run
  *                       parts that have a mixed-precision variant (x-tiled,
  *                       sliced BSR); the others stay FP32;
@@ -394,7 +398,12 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  *                       whose codec_bound (clspmv_part_info) exceeds it is
  *                       built in FP32 instead;
  *   "cocktail_graph"    0 (default): do_spmv and power_iterate launch each
- *                       part directly; 1: they replay captured hipGraphs. */
+ *                       part directly; 1: they replay captured hipGraphs;
+ *   "cocktail_streams"  1 (default): do_spmv runs the row runs of a plan
+ *                       side by side on up to four streams; 0: one stream;
+ *   "cocktail_small_plan" entries (default 4194304) below which merged flat
+ *                       rows that fall into several runs run as one part of
+ *                       adaptive row blocks; 0: never. */
 int clspmv_set_option(const char* name, long long value);
 int clspmv_part_get_info(clspmv_part_t p, clspmv_part_info* info);
 /* LDS x-tiled parts (no reference counterpart): tiles per SpMV (one x

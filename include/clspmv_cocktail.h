@@ -108,6 +108,9 @@ int clspmv_init_mat_kernels(clspmv_ctx_t ctx, const clspmv_cocktail* mat, const 
 int clspmv_free_mat_kernels(clspmv_gpumat_t gpumat);
 /* do_spmv (:1088-1132): ntimes * (y += A x), parts in the reference's fixed
  * order BDIA, DIA, SBELL, BELL, BCSR, SELL, ELL, CSR, COO on one stream;
+ * a plan made only of row runs (each part writing rows of its own) runs
+ * its parts side by side on up to four streams, forked from and joined
+ * back to the context stream ("cocktail_streams" 0: one stream);
  * asynchronous (clspmv_synchronize waits).  With
  * clspmv_set_option("cocktail_graph", 1) the passes are replayed from
  * hipGraphs captured on first use (up to 64 passes per graph launch);

@@ -2,6 +2,8 @@
 do_spmv, compared with the reference's CPU SpMV restated in oracle/ (same
 bound as the single-format tests), on matrices whose partition has one to
 three parts; plus the y = A^k x iteration of A/src/eval.cpp:19-72."""
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -64,6 +66,16 @@ def test_cocktail_mi355x_tables(ctx, merge_flat, case):
     gpu.free()
 
 
+@contextlib.contextmanager
+def _option(name, value, default):
+    """clspmv_set_option(name, value) for the block, then back to default."""
+    L.check(L.lib().clspmv_set_option(name, value), "set_option")
+    try:
+        yield
+    finally:
+        L.check(L.lib().clspmv_set_option(name, default), "set_option")
+
+
 def _local_scattered(width, seed):
     """3 x 32768 rows, 8 entries each: the first and last 32768 rows within
     2000 columns of the diagonal, the middle ones uniform over the width."""
@@ -81,9 +93,11 @@ def _local_scattered(width, seed):
 
 @pytest.mark.parametrize("width", [1 << 20, 1 << 22], ids=["x_in_l2", "wide_x"])
 def test_cocktail_split_local(ctx, tables, width):
-    # the merged flat rows become three row-run parts (local, scattered,
-    # local) launched with y offset to their first rows; same product as one
-    # part
+    # wide x: the merged flat rows become three row-run parts (local,
+    # scattered, local) launched with y offset to their first rows; same
+    # product as one part.  x within L2: the rows, all of one length, are one
+    # SELL-class run wherever their columns lie, so the partition's own flat
+    # part stays
     coo = _local_scattered(width, 31)
     mat = CK.analyze_matrix(coo.copy(), tables)
     flat = [n for n in mat.parts() if n in ("sell", "ell", "csr", "coo")]
@@ -93,14 +107,13 @@ def test_cocktail_split_local(ctx, tables, width):
     y0 = M.gen_vector(coo.height, 26)
     ys = {}
     for split in (1, 0):
-        L.check(L.lib().clspmv_set_option(b"cocktail_split_local", split), "set_option")
-        try:
+        # (runs at every size: 786432 entries are under the one-part size)
+        with _option(b"cocktail_split_local", split, 1), _option(b"cocktail_small_plan", 0, 1 << 22):
             gpu = CK.GpuCocktail(ctx, mat, x, y0)
-        finally:
-            L.check(L.lib().clspmv_set_option(b"cocktail_split_local", 1), "set_option")
         parts = gpu.part_infos()
-        assert len(parts) == other + (3 if split else 1), (split, mat.parts(), parts)
-        if split:
+        runs = 3 if split and width > 1 << 20 else 1
+        assert len(parts) == other + runs, (split, mat.parts(), parts)
+        if runs == 3:
             assert sorted(p["rows"] for p in parts[other:]) == [32768, 32768, 32768]
         gpu.do_spmv(1)
         ys[split] = gpu.result_to_cpu()
@@ -109,11 +122,11 @@ def test_cocktail_split_local(ctx, tables, width):
         gpu.free()
 
 
-def _blocks_scattered(seed):
-    """2 x 32768 rows of width 2^22: dense aligned 4 x 4 blocks (6 per block
+def _blocks_scattered(seed, B=32768):
+    """2 x B rows of width 2^22: dense aligned 4 x 4 blocks (6 per block
     row, near the diagonal), then 10 uniform columns per row."""
     rng = np.random.default_rng(seed)
-    B, W = 32768, 1 << 22
+    W = 1 << 22
     br = np.arange(B // 4)
     bc = np.clip(br[:, None] + rng.integers(-1000, 1001, (B // 4, 6)), 0, W // 4 - 1)
     rows = [(br[:, None] * 4 + i).repeat(6, 1).ravel() for i in range(4) for _ in range(4)]
@@ -126,16 +139,18 @@ def _blocks_scattered(seed):
 
 
 def test_cocktail_runs_by_shape(ctx, tables):
-    # local rows of dense 4 x 4 blocks run as sliced BSR, local rows of equal
-    # length as SELL, scattered rows x-tiled or column-sliced
-    cases = [(_blocks_scattered(41), ["bsr44_sliced", "scattered"]),
+    # local rows of dense 4 x 4 blocks run as BSR (a run of 786432 entries,
+    # under 2^21: 2 x 2 blocks with lanes per block row), local rows of
+    # equal length as SELL, scattered rows x-tiled or column-sliced
+    cases = [(_blocks_scattered(41), ["bsr22_L", "scattered"]),
              (_local_scattered(1 << 22, 43), ["sell", "scattered", "sell"])]
     for coo, kinds in cases:
         mat = CK.analyze_matrix(coo.copy(), tables)
         assert mat.parts() == ["ell"], mat.parts()
         x = M.gen_vector(coo.width, 27)
         y0 = M.gen_vector(coo.height, 28)
-        gpu = CK.GpuCocktail(ctx, mat, x, y0)
+        with _option(b"cocktail_small_plan", 0, 1 << 22):  # runs at every size
+            gpu = CK.GpuCocktail(ctx, mat, x, y0)
         names = [p["kernel"] for p in gpu.part_infos()]
         assert len(names) == len(kinds), names
         for name, kind in zip(names, kinds):
@@ -300,3 +315,34 @@ def test_plan_kernels(ctx, case, tname):
     assert rep["ok"], (case, tname, rep)
     gpu.free()
     plan.free()
+
+
+@pytest.mark.parametrize("rows", [16064, 73024, 262144])
+def test_concurrent_runs_equal_one_stream(ctx, rows):
+    # the row runs of a composite plan (SELL, sliced BSR, CSR) on up to four
+    # streams give y bit for bit as on one stream, over several passes and
+    # two calls, and at oracle parity after one pass
+    coo = M.gen_cocktail_composite(rows, 1000 + rows)
+    with _option(b"cocktail_small_plan", 0, 1 << 22):  # runs at every size
+        plan = CK.Plan.analyze(coo.copy(), CK.BenchTables(CK.MI355X_TABLES))
+    assert len(plan.parts()) >= 3 and all(q["source"] == 9 for q in plan.parts())
+    x = M.gen_vector(coo.width, 43)
+    y0 = M.gen_vector(coo.height, 44)
+    out = {}
+    try:
+        for streams in (0, 1):
+            L.check(L.lib().clspmv_set_option(b"cocktail_streams", streams), "set_option")
+            gpu = CK.GpuCocktail(ctx, plan, x, y0)
+            gpu.do_spmv(1)
+            one = gpu.result_to_cpu()
+            gpu.do_spmv(3)
+            gpu.do_spmv(2)
+            out[streams] = (one, gpu.result_to_cpu())
+            gpu.free()
+    finally:
+        L.check(L.lib().clspmv_set_option(b"cocktail_streams", 1), "set_option")
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    rep = O.parity_report(out[1][0], coo, x, y0)
+    assert rep["ok"], rep
+    plan.free()
+    coo.free()

@@ -120,9 +120,11 @@ def test_bsr_mixed_lds_window_matches_plain(ctx, codec):
     coo = M.gen_block_rows(5000, 3, 9, 2000, 33)
     bsr = M.coo2bsr(coo, 3, 3)
     x, y0 = M.gen_vector(coo.width, 3), M.gen_vector(coo.height, 4)
-    pw = M.MixedPart(ctx, L.FMT_BSR, bsr, codec, True)
-    L.check(L.lib().clspmv_set_option(b"bsr_lds_window", 0), "set_option")
     try:
+        # windows wherever they fit (5 windows: fewer than the default rule's 128)
+        L.check(L.lib().clspmv_set_option(b"bsr_lds_window", 2), "set_option")
+        pw = M.MixedPart(ctx, L.FMT_BSR, bsr, codec, True)
+        L.check(L.lib().clspmv_set_option(b"bsr_lds_window", 0), "set_option")
         pp = M.MixedPart(ctx, L.FMT_BSR, bsr, codec, True)
     finally:
         L.check(L.lib().clspmv_set_option(b"bsr_lds_window", 1), "set_option")
@@ -273,13 +275,15 @@ def test_e4m3_decode_every_code(ctx):
 @pytest.mark.parametrize("codec", CODECS)
 def test_cocktail_value_codec(ctx, codec):
     # the Cocktail row-run parts with a mixed-precision variant (sliced BSR,
-    # x-tiled) take the codec; the others stay FP32
+    # x-tiled) take the codec; the others stay FP32.  131072 block rows: a
+    # BSR run of 3.1M entries, sliced (a run under 2^21 entries takes the
+    # lanes layout, which has no codec variant)
     import ctypes as C
 
     from clspmv_amd import cocktail as CK
     from tests.test_gpu_cocktail import _blocks_scattered
 
-    coo = _blocks_scattered(41)
+    coo = _blocks_scattered(41, 131072)
     mat = CK.analyze_matrix(coo.copy(), CK.BenchTables(CK.REFERENCE_TABLES))
     x, y0 = M.gen_vector(coo.width, 29), M.gen_vector(coo.height, 30)
     L.check(L.lib().clspmv_set_option(b"cocktail_value_codec", C.c_longlong(codec)), "set_option")
@@ -343,7 +347,7 @@ def test_cocktail_codec_guard(ctx, codec):
     from clspmv_amd import cocktail as CK
     from tests.test_gpu_cocktail import _blocks_scattered
 
-    base = _blocks_scattered(41)
+    base = _blocks_scattered(41, 131072)  # a sliced BSR run (see test_cocktail_value_codec)
     rng = np.random.default_rng(5)
     ints = _with_values(base, rng.choice([-4.0, -2.0, -1.0, 1.0, 2.0, 4.0], base.nnz))
     planted_vals = ints.data.copy()

@@ -315,7 +315,7 @@ def test_set_option_ranges():
     cases = {b"col_slice_bytes": ([256, 1 << 21], [0, 255], 1 << 21),
              b"xtile_blocks": ([1, 256, 1 << 20], [0, (1 << 20) + 1], 256),
              b"xtile_max_groups": ([1, 8], [0, 9], 8),
-             b"bsr_lds_window": ([0, 1], [-1, 2], 1),
+             b"bsr_lds_window": ([0, 1, 2], [-1, 3], 1),
              b"cocktail_value_codec": ([0, L.VAL_BF16X2, L.VAL_FP16_ROW, L.VAL_FP8_SLICE, L.VAL_I16_ROW],
                                        [-1, L.VAL_I16_ROW + 1], 0)}
     try:

@@ -76,3 +76,46 @@ def test_band_and_block_runs(tables):
         coo = AC.make(dict(AC.CASES)[case])
         parts = CK.Plan.analyze(coo.copy(), tables["mi355x"]).parts()
         assert [(q["source"], q["format"], q["row0"], q["row1"]) for q in parts] == [(9, fmt, 0, coo.height)], case
+
+
+@pytest.mark.parametrize("rows", [16064, 23824, 73024, 358448])
+def test_composite_runs_follow_unaligned_boundaries(tables, rows):
+    # band rows [0, q1), 4 x 4 block rows [q1, q2), Zipf rows [q2, rows) with
+    # q1, q2 not on a run block: each boundary lands on the 128-row step
+    # next to it, the step holding both kinds of rows staying out of the BSR
+    # run, so that run holds block rows only (24 entries each)
+    from clspmv_amd import matrix as M
+    coo = M.gen_cocktail_composite(rows, 1000 + rows)
+    L.check(L.lib().clspmv_set_option(b"cocktail_small_plan", 0), "set_option")  # runs at every size
+    try:
+        p = CK.Plan.analyze(coo.copy(), tables["mi355x"])
+    finally:
+        L.check(L.lib().clspmv_set_option(b"cocktail_small_plan", 1 << 22), "set_option")
+    parts = p.parts()
+    q1, q2 = rows // 4, rows // 2
+    assert [q["format"] for q in parts] == [L.FMT_SELL, L.FMT_BSR, L.FMT_CSR_VECTOR], parts
+    bsr = parts[1]
+    assert bsr["row0"] == -(-q1 // 128) * 128 and bsr["row1"] == q2 // 128 * 128, (q1, q2, parts)
+    assert bsr["nnz"] == 24 * (bsr["row1"] - bsr["row0"])
+    _check_cover(parts, coo)
+    p.free()
+    coo.free()
+
+
+def test_small_matrix_runs_as_one_part(tables):
+    # under 2^22 entries a composite's runs become one adaptive CSR part;
+    # from there on the runs stay (cocktail_small_plan sets the size)
+    from clspmv_amd import matrix as M
+    small = M.gen_cocktail_composite(73024, 7)
+    assert small.nnz < 1 << 22
+    parts = CK.Plan.analyze(small.copy(), tables["mi355x"]).parts()
+    assert [(q["source"], q["format"], q["method"], q["row0"], q["row1"]) for q in parts] == [
+        (9, L.FMT_CSR_VECTOR, 1, 0, small.height)]
+    L.check(L.lib().clspmv_set_option(b"cocktail_small_plan", small.nnz + 1), "set_option")
+    try:
+        assert len(CK.Plan.analyze(small.copy(), tables["mi355x"]).parts()) == 1
+        L.check(L.lib().clspmv_set_option(b"cocktail_small_plan", small.nnz), "set_option")
+        assert len(CK.Plan.analyze(small.copy(), tables["mi355x"]).parts()) == 3
+    finally:
+        L.check(L.lib().clspmv_set_option(b"cocktail_small_plan", 1 << 22), "set_option")
+    small.free()
