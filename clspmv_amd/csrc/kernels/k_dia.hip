@@ -111,42 +111,32 @@ __global__ __launch_bounds__(256) void dia_row4_kernel(const int* __restrict__ o
     const bool interior = wrow0 + offmin >= 0 && wrow0 + 1023 + offmax < cols;
     if (row >= rows) return;
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    if (interior) {
-        int d = 0;
-        for (; d + 4 <= ndiag; d += 4) {
-            float4 v[4];
-            float xv[4][4];
+    // 6 diagonals per step (the last step's missing ones skipped, uniform
+    // test); boundary workgroups take the same steps with bounds-tested x
+    for (int d = 0; d < ndiag; d += 6) {
+        float4 v[6];
+        float xv[6][4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 6; ++u)
+            if (d + u < ndiag) {
                 v[u] = ld_stream(data + (size_t)(d + u) * L4 + r4);
-                const f32x4a4 t = *reinterpret_cast<const f32x4a4*>(x + row + offsets[d + u]);
-                xv[u][0] = t.x, xv[u][1] = t.y, xv[u][2] = t.z, xv[u][3] = t.w;
+                const int c = row + offsets[d + u];
+                if (interior) {
+                    const f32x4a4 t = *reinterpret_cast<const f32x4a4*>(x + c);
+                    xv[u][0] = t.x, xv[u][1] = t.y, xv[u][2] = t.z, xv[u][3] = t.w;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) xv[u][j] = xload(x, c + j, cols);
+                }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 6; ++u)
+            if (d + u < ndiag) {
                 a0 = fmaf(v[u].x, xv[u][0], a0);
                 a1 = fmaf(v[u].y, xv[u][1], a1);
                 a2 = fmaf(v[u].z, xv[u][2], a2);
                 a3 = fmaf(v[u].w, xv[u][3], a3);
             }
-        }
-        for (; d < ndiag; ++d) {
-            const float4 v = ld_stream(data + (size_t)d * L4 + r4);
-            const f32x4a4 t = *reinterpret_cast<const f32x4a4*>(x + row + offsets[d]);
-            a0 = fmaf(v.x, t.x, a0);
-            a1 = fmaf(v.y, t.y, a1);
-            a2 = fmaf(v.z, t.z, a2);
-            a3 = fmaf(v.w, t.w, a3);
-        }
-    } else {
-        for (int d = 0; d < ndiag; ++d) {
-            const int c = row + offsets[d];
-            const float4 v = ld_stream(data + (size_t)d * L4 + r4);
-            a0 = fmaf(v.x, xload(x, c, cols), a0);
-            a1 = fmaf(v.y, xload(x, c + 1, cols), a1);
-            a2 = fmaf(v.z, xload(x, c + 2, cols), a2);
-            a3 = fmaf(v.w, xload(x, c + 3, cols), a3);
-        }
     }
     if (row + 3 < rows) {
         float4 yo = *reinterpret_cast<const float4*>(y + row);

@@ -180,39 +180,35 @@ __global__ __launch_bounds__(256) void dia_mixed_kernel(const int* __restrict__ 
     float a[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) a[j] = 0.f;
-    if (interior) {
-        // U diagonals' loads in flight per step, the last step's missing ones
-        // skipped by a wave-uniform test (no serial remainder loop); the
-        // sums keep diagonal order
-        constexpr int U = R >= 4 ? 6 : 8;
-        for (int d = 0; d < ndiag; d += U) {
-            unsigned w[U][W];
-            float xv[U][R];
+    // U diagonals' loads in flight per step, the last step's missing ones
+    // skipped by a wave-uniform test (no serial remainder loop); the sums
+    // keep diagonal order.  Boundary workgroups take the same steps with
+    // bounds-tested x loads: a one-diagonal-at-a-time loop there made those
+    // few workgroups the kernel's tail.
+    constexpr int U = R >= 4 ? 6 : 8;
+    for (int d = 0; d < ndiag; d += U) {
+        unsigned w[U][W];
+        float xv[U][R];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (d + u < ndiag) {
-                    word(d + u, w[u]);
-                    load_x_run<R>(x, row + offsets[d + u], xv[u]);
+        for (int u = 0; u < U; ++u)
+            if (d + u < ndiag) {
+                word(d + u, w[u]);
+                const int c = row + offsets[d + u];
+                if (interior) {
+                    load_x_run<R>(x, c, xv[u]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < R; ++j) xv[u][j] = xload_m(x, c + j, cols);
                 }
+            }
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (d + u < ndiag) {
-                    float v[R];
-                    dia_decode<CODEC, R, W>(w[u], slice_scale(d + u), v);
+        for (int u = 0; u < U; ++u)
+            if (d + u < ndiag) {
+                float v[R];
+                dia_decode<CODEC, R, W>(w[u], slice_scale(d + u), v);
 #pragma unroll
-                    for (int j = 0; j < R; ++j) a[j] = fmaf(v[j], xv[u][j], a[j]);
-                }
-        }
-    } else {
-        for (int d = 0; d < ndiag; ++d) {
-            unsigned w[W];
-            word(d, w);
-            float v[R];
-            dia_decode<CODEC, R, W>(w, slice_scale(d), v);
-            const int c = row + offsets[d];
-#pragma unroll
-            for (int j = 0; j < R; ++j) a[j] = fmaf(v[j], xload_m(x, c + j, cols), a[j]);
-        }
+                for (int j = 0; j < R; ++j) a[j] = fmaf(v[j], xv[u][j], a[j]);
+            }
     }
     float rs[R];
     if (codec_row_scaled(CODEC)) load_vec<R>(scale + row, rs);  // the scale array has L >= row + R entries
