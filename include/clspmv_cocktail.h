@@ -136,20 +136,24 @@ int clspmv_gpumat_buffers(clspmv_gpumat_t gpumat, void** d_vec, void** d_res);
 /* ------------------------------------------------------------------ */
 /* MI355X mode of the analyzer: the plan.  A Cocktail partition keeps the
  * reference's nine parts; on MI355X the flat ones (ELL, SELL, CSR, COO) are
- * merged into one set of rows and cut into runs of 32768-row blocks by where
- * their columns lie and their shape -- rows of dense aligned 4x4 (2x2)
- * blocks as sliced BSR, near-equal rows as SELL, other local rows as CSR
- * row blocks, scattered rows in the LDS x-tiled or column-sliced layout by
- * a fitted cost rule.  The plan lists the parts that will run, each with
- * its format, kernel method and row range; clspmv_init_mat_kernels
- * executes the plan of the partition it is given.  Extends analyze_matrix
+ * merged into one set of rows and cut into row runs by class, classified
+ * on blocks of 1024-32768 rows with boundaries placed to 128 rows: rows on
+ * <= 64 well-filled diagonals as DIA, rows of dense aligned 4x4 (2x2)
+ * blocks as BSR, near-equal rows as SELL, other rows as CSR (adaptive row
+ * blocks, or the LDS x-tiled / column-sliced layout for scattered columns);
+ * each run's layout follows its class and size, and a matrix under
+ * "cocktail_small_plan" entries whose rows fall into several runs runs as
+ * one CSR part.  The plan lists the parts that will run, each with its
+ * format, kernel method and row range; clspmv_init_mat_kernels executes
+ * the plan of the partition it is given.  Extends analyze_matrix
  * (A/src/analyze.cpp:3714-3900), whose partition stays bit-exact. */
 typedef struct clspmv_plan_s* clspmv_plan_t;
 typedef struct {
     int source;       /* 0..8: part of the partition (bdia, dia, sbell, bell, bcsr, sell, ell, csr, coo);
                          9: a row run of its merged flat rows */
     int format;       /* CLSPMV_FMT_* the part runs as */
-    int method;       /* the method its clspmv_part_* call receives (4: LDS x-tiled, 2: column-sliced) */
+    int method;       /* the method its clspmv_part_* call receives (CSR 4: LDS x-tiled, 2: column-sliced,
+                         1: adaptive; BSR 0: sliced, 1: lanes per block row; DIA 4: lanes per row) */
     int row0, row1;   /* rows it covers */
     long long nnz;
 } clspmv_plan_part;
