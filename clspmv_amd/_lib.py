@@ -88,7 +88,7 @@ class Bsr(C.Structure):
 
 class PlanPart(C.Structure):
     _fields_ = [("source", C.c_int), ("format", C.c_int), ("method", C.c_int), ("row0", C.c_int), ("row1", C.c_int),
-                ("nnz", C.c_longlong)]
+                ("nnz", C.c_longlong), ("block", C.c_int)]
 
 
 class BatchEntry(C.Structure):

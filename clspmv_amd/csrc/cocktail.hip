@@ -952,7 +952,8 @@ int clspmv_plan_num_parts(clspmv_plan_t P) { return P ? (int)P->parts.size() : 0
 int clspmv_plan_get_part(clspmv_plan_t P, int i, clspmv_plan_part* out) {
     if (!P || !out || i < 0 || i >= (int)P->parts.size()) return CLSPMV_E_INVALID;
     const PlanPart& q = P->parts[(size_t)i];
-    *out = clspmv_plan_part{q.source, q.format, q.method, q.row0, q.row1, q.nnz};
+    const int block = q.source == 9 && q.format == CLSPMV_FMT_BSR ? (int)run_block(q.cls, q.method) : 0;
+    *out = clspmv_plan_part{q.source, q.format, q.method, q.row0, q.row1, q.nnz, block};
     return CLSPMV_OK;
 }
 

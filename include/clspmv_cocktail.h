@@ -156,6 +156,8 @@ typedef struct {
                          1: adaptive; BSR 0: sliced, 1: lanes per block row; DIA 4: lanes per row) */
     int row0, row1;   /* rows it covers */
     long long nnz;
+    int block;        /* BSR row runs: the square block size of the part (its class's, or 2 for 4x4
+                         blocks run with lanes per block row); 0 otherwise */
 } clspmv_plan_part;
 /* analyze_matrix (these tables and knobs) and the plan of its partition;
  * the plan owns the partition. */

@@ -54,6 +54,7 @@ def test_composite_plan_follows_its_structure(tables, tname):
     assert [(q["format"], q["row0"], q["row1"]) for q in parts] == [
         (L.FMT_SELL, 0, M1), (L.FMT_BSR, M1, 2 * M1), (L.FMT_CSR_VECTOR, 2 * M1, 4 * M1)]
     assert parts[2]["method"] == 4  # LDS x-tiled
+    assert [q["block"] for q in parts] == [0, 4, 0]  # sliced BSR 4 x 4
     assert parts[1]["nnz"] == M1 // 4 * 6 * 16  # 6 dense 4 x 4 blocks per block row
     _check_cover(parts, coo)
     p.free()
@@ -76,6 +77,7 @@ def test_band_and_block_runs(tables):
         coo = AC.make(dict(AC.CASES)[case])
         parts = CK.Plan.analyze(coo.copy(), tables["mi355x"]).parts()
         assert [(q["source"], q["format"], q["row0"], q["row1"]) for q in parts] == [(9, fmt, 0, coo.height)], case
+        assert parts[0]["block"] == (3 if fmt == L.FMT_BSR else 0), parts
 
 
 @pytest.mark.parametrize("rows", [16064, 23824, 73024, 358448])
@@ -97,6 +99,8 @@ def test_composite_runs_follow_unaligned_boundaries(tables, rows):
     bsr = parts[1]
     assert bsr["row0"] == -(-q1 // 128) * 128 and bsr["row1"] == q2 // 128 * 128, (q1, q2, parts)
     assert bsr["nnz"] == 24 * (bsr["row1"] - bsr["row0"])
+    # 4 x 4 blocks: sliced from 2^21 entries, below with lanes per block row as 2 x 2 blocks
+    assert (bsr["method"], bsr["block"]) == ((0, 4) if bsr["nnz"] >= 1 << 21 else (1, 2)), bsr
     _check_cover(parts, coo)
     p.free()
     coo.free()
