@@ -27,6 +27,9 @@ def main(argv=None):
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--methods", default="", help="kernel variant overrides, e.g. c1b=2,c1c=2")
     ap.add_argument("--scale", type=int, default=1)
+    ap.add_argument("--batch-blocks", type=int, default=0, metavar="N",
+                    help="also run one batch of N 3x3-block matrices (BSR entries, k = 16: the matrix-core "
+                         "kernel) ntimes + 1 times")
     args = ap.parse_args(argv)
     ctx = M.Context(0)
     out = {"kernels": [], "copy": None}
@@ -44,7 +47,9 @@ def main(argv=None):
     for item in filter(None, args.methods.split(",")):
         k, v = item.split("=")
         overrides.setdefault(k, []).append(int(v))
-    for name in args.matrices.split(","):
+    if args.batch_blocks:
+        out["batch"] = _run_batch(ctx, args.batch_blocks, args.ntimes)
+    for name in filter(None, args.matrices.split(",")):
         wl = W.build(name, args.scale)
         for meth in overrides.get(name, [wl.method]):
             _run_one(ctx, out, name, wl, meth, args.ntimes)
@@ -72,6 +77,36 @@ def _run_one(ctx, out, name, wl, meth, ntimes):
     part.free()
     dx.free()
     dy.free()
+
+
+def _run_batch(ctx, n, ntimes, k=16):
+    """One batch of n matrices of 3x3 blocks (9 per block row, 65536 block
+    rows each) as BSR entries with k = 16 right-hand sides: the batched
+    kernel's block items on the matrix cores."""
+    from . import batch as B
+    coos = [M.gen_block_rows(65536, 3, 9, 5000, 900 + i) for i in range(n)]
+    mats = [M.coo2bsr(c, 3, 3) for c in coos]
+    b = B.Batch(ctx, mats, [k] * n)
+    for i, c in enumerate(coos):
+        b.set_x(i, np.tile(M.gen_vector(c.width, 30 + i)[:, None], (1, k)))
+    b.run(1)
+    ctx.sync()
+    ctx.time_begin()
+    b.run(ntimes)
+    ms = ctx.time_end() / ntimes
+    st = b.stats()
+    flops = 2.0 * st["nnz_rhs"]
+    res = {"matrices": n, "k": k, "nnz": st["nnz"], "nnz_rhs": st["nnz_rhs"], "alg_bytes": st["alg_bytes"],
+           "runs": ntimes + 1, "us": round(ms * 1e3, 2), "tflops": round(flops / (ms * 1e-3) / 1e12, 3),
+           "gbps_alg": round(st["alg_bytes"] / (ms * 1e-3) / 1e9, 1)}
+    print(f"[kprof] batch {n} x 3x3 blocks k={k}: {ms * 1e3:.1f} us, {res['tflops']} TFLOP/s, "
+          f"{res['gbps_alg']} GB/s", file=sys.stderr, flush=True)
+    b.free()
+    for m in mats:
+        m.free()
+    for c in coos:
+        c.free()
+    return res
 
 
 if __name__ == "__main__":

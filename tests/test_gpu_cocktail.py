@@ -346,3 +346,31 @@ def test_concurrent_runs_equal_one_stream(ctx, rows):
     assert rep["ok"], rep
     plan.free()
     coo.free()
+
+
+def test_plan_parity_on_the_selector_family(ctx):
+    # 60 matrices of the autotuner's family (stencils, Laplacians, uniform,
+    # Zipf and block rows, composites; 2^10..2^16 rows) through the
+    # analyzer's MI355X plan -- every run class and layout the plan picks
+    # (DIA / BSR lanes, SELL, x-tiled, one part for small composites, runs
+    # side by side on streams) -- at oracle parity after one pass
+    from clspmv_amd import selector as S
+    tables = CK.BenchTables(CK.MI355X_TABLES)
+    rng = np.random.default_rng(11)
+    seen = set()
+    for i in range(60):
+        kind, coo = S.random_matrix(i, rng, 10, 16)
+        plan = CK.Plan.analyze(coo.copy(), tables)
+        x = M.gen_vector(coo.width, 100 + i)
+        y0 = M.gen_vector(coo.height, 200 + i)
+        gpu = CK.GpuCocktail(ctx, plan, x, y0)
+        names = [p["kernel"] for p in gpu.part_infos()]
+        seen.update(n.split("_")[0] + ("_lanes" if "lanes" in n or "_L" in n else "") for n in names)
+        gpu.do_spmv(1)
+        rep = O.parity_report(gpu.result_to_cpu(), coo, x, y0)
+        assert rep["ok"], (i, kind, coo.height, names, rep)
+        gpu.free()
+        plan.free()
+        coo.free()
+    # the family reaches the small-matrix layouts
+    assert {"dia_lanes", "bsr22_lanes"} & seen, seen
