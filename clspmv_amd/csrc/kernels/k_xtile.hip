@@ -235,13 +235,15 @@ __device__ __forceinline__ f32x2 xt_decode(typename XtVal<CODEC>::T v, float s) 
 }
 
 // Registers holding one tile ahead of use: its x slice (unless the tile
-// continues the previous tile's slice) and this wavefront's chunks.
+// continues the previous tile's slice), this wavefront's chunks, and the
+// tile's metadata (wave-uniform), read when the tile was fetched.
 template <int CODEC>
 struct XtSet {
     f32x4u x[kXtXLoads];
     u32x2 e[kXtRounds];
     typename XtVal<CODEC>::T v[kXtRounds];
     float s[kXtRounds];  // FP8 chunk scales
+    int ts, c0, c1;      // slice word (slice << 1 | continues) and chunk range
 };
 
 struct XtArgs {
@@ -261,16 +263,18 @@ struct XtArgs {
 // register set is refilled in place right after its last use, so that the
 // wait before each use covers only the loads issued before it.
 template <int CODEC>
-__device__ __forceinline__ void xt_fetch_x(XtSet<CODEC>& S, const XtArgs& a, int t, int tid) {
-    const int base = (a.tile_slice[t] >> 1) * kXtCols;
+__device__ __forceinline__ void xt_fetch_x(XtSet<CODEC>& S, const XtArgs& a, int ts, int tid) {
+    S.ts = ts;
+    const int base = (ts >> 1) * kXtCols;
 #pragma unroll
     for (int q = 0; q < kXtXLoads; ++q)
         S.x[q] = load_x4(a.x, base + min(q * kXtThreads + tid, kXtX4 - 1) * 4, a.cols);
 }
 template <int CODEC>
-__device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int t, int wave, int lane) {
+__device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int c0, int c1, int wave, int lane) {
     typedef typename XtVal<CODEC>::T V;
-    const int c0 = a.tile_chunk[t], c1 = a.tile_chunk[t + 1];
+    S.c0 = c0;
+    S.c1 = c1;
 #pragma unroll
     for (int k = 0; k < kXtRounds; ++k) {
         // wave-uniform chunk index (the FP8 scale becomes a scalar load);
@@ -295,6 +299,37 @@ struct XtState {
     int pending;  // bit k: the carry set of chunk round k awaits its addition
 };
 
+// Metadata of 64 consecutive tiles of the workgroup, lane l holding tile
+// base + l, read back with a wave-uniform lane index: a tile's metadata
+// costs no memory round trip (scalar loads of it, waited on before each
+// barrier with the LDS operations, were on every tile's critical path);
+// the block is reloaded every 64 tiles.
+struct XtMeta {
+    int base;
+    int ts, c0, c1;  // slice word, chunk range of tile base + lane
+};
+__device__ __forceinline__ void xt_meta_load(XtMeta& M, const XtArgs& a, int base, int t1, int lane) {
+    M.base = base;
+    const int t = min(base + lane, t1 - 1);
+    M.ts = a.tile_slice[t];
+    M.c0 = a.tile_chunk[t];
+    M.c1 = a.tile_chunk[t + 1];
+    // wait here, on the reload path only (vmcnt(0); expcnt and lgkmcnt
+    // untouched): otherwise the compiler waits for every load in flight at
+    // the reads after the branch, on every tile
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+}
+// Metadata of tile t (wave-uniform, t >= M.base; tiles are asked for in
+// increasing order).
+__device__ __forceinline__ void xt_meta(XtMeta& M, const XtArgs& a, int t, int t1, int lane, int& ts, int& c0,
+                                        int& c1) {
+    if (t - M.base >= 64) xt_meta_load(M, a, t, t1, lane);
+    const int i = t - M.base;
+    ts = __builtin_amdgcn_readlane(M.ts, i);
+    c0 = __builtin_amdgcn_readlane(M.c0, i);
+    c1 = __builtin_amdgcn_readlane(M.c1, i);
+}
+
 // One tile: stage its x slice from set S, refill S with tile t + kXtDepth,
 // reduce its chunks.  Steps past the block's last tile (t >= t1) only keep
 // the barrier sequence and the carry bookkeeping, so that the unrolled loop
@@ -304,15 +339,19 @@ struct XtState {
 // chunk), its other chunks' sums go to their carry slots, and each round
 // has its own carry set, added after the next tile's first barrier.
 template <int CODEC>
-__device__ __forceinline__ void xt_step(XtSet<CODEC>& S, const XtArgs& a, int t, int t1, float* y_l, float4* x_l4,
-                                        float* dummy, int (*c_row)[kXtWaves], float (*c_val)[kXtWaves], XtState& st,
-                                        int tid, int wave, int lane) {
+__device__ __forceinline__ void xt_step(XtSet<CODEC>& S, XtMeta& M, const XtArgs& a, int t, int t1, float* y_l,
+                                        float4* x_l4, float* dummy, int (*c_row)[kXtWaves],
+                                        float (*c_val)[kXtWaves], XtState& st, int tid, int wave, int lane) {
+    // the metadata of the tile this step prefetches (the current tile's
+    // came with its register set)
+    const int tn = min(t + kXtDepth, t1 - 1);
+    int ts_n, c0_n, c1_n;
+    xt_meta(M, a, tn, t1, lane, ts_n, c0_n, c1_n);
     __syncthreads();  // previous tile done with x_l; its carry slots are written
     if (st.pending && wave == 0) xt_apply_carries(st.pending, c_row, c_val, y_l, lane);
     st.pending = 0;
     const bool live = t < t1;
-    const int tt = live ? t : t1 - 1;
-    const int ts = a.tile_slice[tt];
+    const int ts = S.ts;  // tile min(t, t1 - 1)
     if (live && !(ts & 1)) {
         const int base = (ts >> 1) * kXtCols;
         if (base + kXtCols <= a.cols) {
@@ -330,9 +369,8 @@ __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, const XtArgs& a, int t,
             }
         }
     }
-    const int c0 = a.tile_chunk[tt], c1 = live ? a.tile_chunk[tt + 1] : c0;
-    const int tn = min(t + kXtDepth, t1 - 1);
-    xt_fetch_x(S, a, tn, tid);
+    const int c0 = S.c0, c1 = live ? S.c1 : c0;
+    xt_fetch_x(S, a, ts_n, tid);
     __syncthreads();  // x_l holds the slice; carries applied
     const float* x_l = reinterpret_cast<const float*>(x_l4);
 #pragma unroll
@@ -346,7 +384,7 @@ __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, const XtArgs& a, int t,
             st.pending |= 1 << k;
         }
     }
-    xt_fetch_e(S, a, tn, wave, lane);
+    xt_fetch_e(S, a, c0_n, c1_n, wave, lane);
 }
 
 template <int CODEC>
@@ -379,16 +417,20 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     // kXtDepth register sets, indexed only by constants (the loops over them
     // are unrolled), so they stay in registers
     XtSet<CODEC> S[kXtDepth];
+    XtMeta M;
+    xt_meta_load(M, a, t0, t1, lane);
 #pragma unroll
     for (int u = 0; u < kXtDepth; ++u) {
-        xt_fetch_x(S[u], a, min(t0 + u, t1 - 1), tid);
-        xt_fetch_e(S[u], a, min(t0 + u, t1 - 1), wave, lane);
+        int ts, c0, c1;
+        xt_meta(M, a, min(t0 + u, t1 - 1), t1, lane, ts, c0, c1);
+        xt_fetch_x(S[u], a, ts, tid);
+        xt_fetch_e(S[u], a, c0, c1, wave, lane);
     }
     XtState st{0};
     for (int t = t0; t < t1; t += kXtDepth) {
 #pragma unroll
         for (int u = 0; u < kXtDepth; ++u)
-            xt_step(S[u], a, t + u, t1, y_l, x_l4, dummy, c_row, c_val, st, tid, wave, lane);
+            xt_step(S[u], M, a, t + u, t1, y_l, x_l4, dummy, c_row, c_val, st, tid, wave, lane);
     }
     __syncthreads();
     if (st.pending && wave == 0) xt_apply_carries(st.pending, c_row, c_val, y_l, lane);
