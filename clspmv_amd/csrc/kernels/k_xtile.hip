@@ -73,16 +73,35 @@ __device__ __forceinline__ float4 x_store4(f32x4u v, int c, int cols) {
 }
 
 
-// Adds one emitted segment sum to y_l -- unless it belongs to the chunk's
-// first row (that one goes to the chunk's carry slot) or to padding -- with
-// no branch: a lane that does not emit here rewrites its own dummy slot
-// instead (the dummies' values are never used; plain loads and stores, so
-// waves touching the same dummy do not serialise as atomics would).  Within
-// one round a row is added to by one lane of one wave only.
-__device__ __forceinline__ void xt_add(bool on, int row, float v, int first_row, float* y_l, float* dummy, int lane) {
-    const bool direct = on && row != first_row && row != (int)kXtPadRow;
-    float* p = direct ? &y_l[row] : &dummy[lane];
-    *p += v;
+// Emits a lane's N segment sums (site i: on[i], row[i], s[i]).  A sum is
+// added to its row of y_l unless it is padding or the chunk's first row
+// continues the previous chunk of the tile (`carry`): that one is stored
+// into the chunk's carry slot.  The additions are LDS read-modify-writes
+// with all N reads issued before the writes (one LDS round trip); a site
+// that adds nothing rewrites the lane's dummy slot instead, so there is no
+// branch (dummy values are never used).  The rows added to are distinct:
+// within one round a row is added to by one lane of one wave only, and the
+// sites of one lane end different segments of a row-sorted chunk.
+template <int N>
+__device__ __forceinline__ void xt_emit(const bool (&on)[N], const int (&row)[N], const float (&s)[N], int first_row,
+                                        bool carry, float* y_l, float* dummy, float* cv, int lane) {
+    float* p[N];
+    float old[N];
+    bool fc = false;
+    float fv = 0.f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const bool first = on[i] && row[i] == first_row;
+        const bool direct = on[i] && row[i] != (int)kXtPadRow && !(carry && first);
+        p[i] = direct ? &y_l[row[i]] : &dummy[lane];
+        fv = first ? s[i] : fv;
+        fc = fc || first;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) old[i] = *p[i];
+#pragma unroll
+    for (int i = 0; i < N; ++i) *p[i] = old[i] + s[i];
+    if (carry && fc) *cv = fv;
 }
 
 // Lane l receives v of lane l - 1 (lane 0 keeps its own): the DPP
@@ -135,27 +154,44 @@ __device__ __forceinline__ void xt_apply_carries(int pending, int (*c_row)[kXtWa
     if (lane < 32 && r >= 0 && (lane == 31 || r_next != r)) y_l[r] += v;
 }
 
-// One wavefront reduces one 128-entry chunk (2 entries per lane).  Adds the
-// sums of segments that start inside the chunk to y_l and writes the
-// chunk's carry slot for its first row.  Each segment is emitted once, by
-// the lane holding its last entry: at most two emit sites per lane in the
-// common case, three in the general one, all evaluated without branches;
-// the first row's sum (exactly one site emits it) is fetched from its lane
-// with a ballot and a readlane.
+// One wavefront reduces one 128-entry chunk (2 entries per lane).  Each
+// segment of equal rows is emitted once, by the lane holding its last
+// entry: at most two emit sites per lane in the common case, three in the
+// general one, all evaluated without branches.  The chunk's first row goes
+// to the chunk's carry slot when its first entry has the carry bit (the row
+// continues the previous chunk of the tile), to y_l otherwise.
 __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, float* y_l, float* dummy, int* cr,
                                          float* cv, int wave, int lane) {
     // products rounded, then added: no multiply-add contraction, so that
     // every value codec (and every compilation) sums the same FP32 numbers
 #pragma clang fp contract(off)
     const int ra = (int)(e.x >> 16), rb = (int)(e.y >> 16);
-    const float pa = v.x * x_l[e.x & 0xFFFFu];
-    const float pb = v.y * x_l[e.y & 0xFFFFu];
+    const float pa = v.x * x_l[e.x & kXtColMask];
+    const float pb = v.y * x_l[e.y & kXtColMask];
     const int head_row = ra;
     const bool brk = ra != rb;
     const float head_sum = pa;
     const int tail_row = rb;
     const float tail_sum = brk ? pb : pa + pb;
-    const int first_row = __builtin_amdgcn_readfirstlane(head_row);  // every lane is active here
+    const int flags = __builtin_amdgcn_readfirstlane((int)e.x);  // every lane is active here
+    if (flags & (int)kXtFastBit) {
+        // No row crosses a lane boundary and none continues the previous
+        // chunk: each lane adds its entries' sums to their rows directly --
+        // entry a alone when the lane's two rows differ, the lane's tail sum
+        // to row b unless it is padding (padding sorts last, so a lane whose
+        // rows differ has a real row a).
+        if (lane == 0) cr[wave] = -1;
+        float* p1 = brk ? &y_l[head_row] : &dummy[lane];
+        float* p2 = tail_row != (int)kXtPadRow ? &y_l[tail_row] : &dummy[lane];
+        const float o1 = *p1, o2 = *p2;
+        *p1 = o1 + head_sum;
+        *p2 = o2 + tail_sum;
+        return;
+    }
+    const int first_row = __builtin_amdgcn_readfirstlane(head_row);
+    const bool carry = (flags & (int)kXtCarryBit) != 0;
+    if (lane == 0) cr[wave] = carry ? first_row : -1;
+    float* slot = &cv[wave];
 
     const int prev_tail_row = lane_up1(tail_row);
     const float prev_tail_sum = lane_up1(tail_sum);
@@ -163,20 +199,16 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
     const bool through = cont && !brk;                        // the whole lane is inside that segment
     const unsigned long long tmask = __ballot(through);
     const unsigned long long cmask = __ballot(cont);
-    // emit sites: (on, row, sum); the third only in the general case
-    bool on1, on2, on3 = false;
-    int row1, row2, row3 = 0;
-    float s1, s2, s3 = 0.f;
     // A segment spans three or more lanes exactly when some lane L lies
     // inside it (through) and lane L + 1 continues it.
     if ((tmask & (cmask >> 1)) == 0) {
         // Every segment spans at most two lanes: one neighbour exchange
         // completes it (the common case for short rows).
         const bool next_cont = lane < 63 && ((cmask >> (lane + 1)) & 1ull);
-        on1 = brk, row1 = head_row, s1 = head_sum + (cont ? prev_tail_sum : 0.f);
-        on2 = !next_cont, row2 = tail_row, s2 = tail_sum + (through ? prev_tail_sum : 0.f);
-        xt_add(on1, row1, s1, first_row, y_l, dummy, lane);
-        xt_add(on2, row2, s2, first_row, y_l, dummy, lane);
+        const bool on[2] = {brk, !next_cont};
+        const int row[2] = {head_row, tail_row};
+        const float sum[2] = {head_sum + (cont ? prev_tail_sum : 0.f), tail_sum + (through ? prev_tail_sum : 0.f)};
+        xt_emit<2>(on, row, sum, first_row, carry, y_l, dummy, slot, lane);
     } else {
         // General case: segmented inclusive scan of the lane tails, all in
         // DPP moves: steps of 1, 2, 4, 8 lanes within each 16-lane row, then
@@ -194,20 +226,10 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
         const float prev_c = lane_up1(cval);
         // a segment that ended exactly before this lane
         const bool boundary_end = lane > 0 && prev_tail_row != head_row;
-        on1 = boundary_end, row1 = prev_tail_row, s1 = prev_c;
-        on2 = brk, row2 = head_row, s2 = head_sum + ((lane > 0 && !boundary_end) ? prev_c : 0.f);
-        on3 = lane == 63, row3 = tail_row, s3 = cval;
-        xt_add(on1, row1, s1, first_row, y_l, dummy, lane);
-        xt_add(on2, row2, s2, first_row, y_l, dummy, lane);
-        xt_add(on3, row3, s3, first_row, y_l, dummy, lane);
-    }
-    const bool f1 = on1 && row1 == first_row, f2 = on2 && row2 == first_row, f3 = on3 && row3 == first_row;
-    const float fv = f1 ? s1 : (f2 ? s2 : s3);
-    const int src = __builtin_ctzll(__ballot(f1 || f2 || f3));  // exactly one lane emits the first row
-    const float first_sum = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fv), src));
-    if (lane == 0) {
-        cr[wave] = first_row;
-        cv[wave] = first_sum;
+        const bool on[3] = {boundary_end, brk, lane == 63};
+        const int row[3] = {prev_tail_row, head_row, tail_row};
+        const float sum[3] = {prev_c, head_sum + ((lane > 0 && !boundary_end) ? prev_c : 0.f), cval};
+        xt_emit<3>(on, row, sum, first_row, carry, y_l, dummy, slot, lane);
     }
 }
 
@@ -243,13 +265,13 @@ struct XtSet {
     u32x2 e[kXtRounds];
     typename XtVal<CODEC>::T v[kXtRounds];
     float s[kXtRounds];  // FP8 chunk scales
-    int ts, c0, c1;      // slice word (slice << 1 | continues) and chunk range
+    int ts, c0, c1;      // slice word and chunk range
 };
 
 struct XtArgs {
     const int* blk_row;
     const int* blk_tile;
-    const int* tile_slice;  // (slice << 1) | continues-previous-slice
+    const int* tile_slice;  // (slice << 2) | 2 if the tile has a carry | 1 if it continues the previous slice
     const int* tile_chunk;
     const u32x2* idx2;
     const void* val;      // chunk values, XtVal<CODEC>::T per lane
@@ -265,7 +287,7 @@ struct XtArgs {
 template <int CODEC>
 __device__ __forceinline__ void xt_fetch_x(XtSet<CODEC>& S, const XtArgs& a, int ts, int tid) {
     S.ts = ts;
-    const int base = (ts >> 1) * kXtCols;
+    const int base = (ts >> 2) * kXtCols;
 #pragma unroll
     for (int q = 0; q < kXtXLoads; ++q)
         S.x[q] = load_x4(a.x, base + min(q * kXtThreads + tid, kXtX4 - 1) * 4, a.cols);
@@ -353,7 +375,7 @@ __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, XtMeta& M, const XtArgs
     const bool live = t < t1;
     const int ts = S.ts;  // tile min(t, t1 - 1)
     if (live && !(ts & 1)) {
-        const int base = (ts >> 1) * kXtCols;
+        const int base = (ts >> 2) * kXtCols;
         if (base + kXtCols <= a.cols) {
 #pragma unroll
             for (int q = 0; q < kXtXLoads; ++q) {
@@ -381,7 +403,7 @@ __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, XtMeta& M, const XtArgs
                 xt_chunk(S.e[k], xt_decode<CODEC>(S.v[k], S.s[k]), x_l, y_l, dummy, c_row[k], c_val[k], wave, lane);
             else if (lane == 0)
                 c_row[k][wave] = -1;
-            st.pending |= 1 << k;
+            if (ts & 2) st.pending |= 1 << k;  // the tile has a carry
         }
     }
     xt_fetch_e(S, a, c0_n, c1_n, wave, lane);

@@ -117,6 +117,9 @@ constexpr bool codec_row_scaled(int c) { return c == kCodecFp16Row || c == kCode
 // LDS x-tiled layout (k_xtile.hip): row blocks of <= kXtRows rows, column
 // slices of kXtCols columns, entries packed (row_in_block << 16 |
 // col_in_slice) in chunks of kXtChunk; padding entries carry row kXtPadRow.
+// The column half of a chunk's first entry carries the chunk's flags:
+// kXtCarryBit when its first row continues the previous chunk of the same
+// tile, kXtFastBit when no row crosses one of its lane boundaries.
 constexpr int kXtThreads = 1024;
 constexpr int kXtWaves = kXtThreads / 64;
 // LDS: y partial sums of kXtRows rows (128 KiB) + one x slice of kXtCols
@@ -131,11 +134,16 @@ constexpr int kXtRounds = 2;                       // chunk rounds per tile held
 constexpr int kXtTileChunks = kXtRounds * kXtWaves;  // a slice with more chunks is split
 constexpr int kXtDepth = 3;                        // tiles prefetched ahead
 constexpr unsigned kXtPadRow = 0xFFFFu;
+constexpr unsigned kXtCarryBit = 0x8000u;
+constexpr unsigned kXtFastBit = 0x4000u;
+constexpr unsigned kXtColMask = 0x3FFFu;
+static_assert(kXtCols <= (int)kXtColMask + 1, "packed entry fields");
 constexpr int kXtMaxGroups = 8;    // column groups per row block at most
 struct DevXtile {
     const int* blk_row;     // nblocks + 1 row boundaries
     const int* blk_tile;    // nblocks + 1 tile ranges
-    const int* tile_slice;  // (column slice << 1) | 1 if the tile continues the previous tile's slice
+    const int* tile_slice;  // (column slice << 2) | 2 if a chunk of the tile has the carry bit
+                            //   | 1 if the tile continues the previous tile's slice
     const int* tile_chunk;  // ntiles + 1 chunk ranges, at most kXtTileChunks per tile
     const unsigned* idx;
     const void* val;        // FP32 values, or encoded with `codec` (k_xtile.hip)

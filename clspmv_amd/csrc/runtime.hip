@@ -731,6 +731,41 @@ int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int
 
 int g_xt_blocks = 256;  // target workgroups: one per CU (tunable: "xtile_blocks")
 
+// Where a row's L entries in one column slice start, given the slice's next
+// free position p (slices start on chunk boundaries, so p mod kXtChunk is
+// the offset in the chunk): a row of 2 entries starts at an even offset, so
+// that one lane holds both, and a row of 3 to kXtAlignMax entries that would
+// straddle a chunk boundary starts the next chunk, so that its sum needs no
+// carry.  The positions skipped are padding.
+constexpr int kXtAlignMax = 16;
+inline long long xt_place(long long p, long long L) {
+    const long long off = p % kXtChunk;
+    if (L == 2) return p + (off & 1);
+    if (L > 2 && L <= kXtAlignMax && off + L > kXtChunk) return p + (kXtChunk - off);
+    return p;
+}
+
+// Visits the rows r0 .. r1 - 1 in order; for each row, first calls
+// place(slice, L) once for every column slice in which the row has L > 0
+// entries (in order of first appearance), then entry(k, slice) for each of
+// its entries.  rowcnt (ns zeros) and touched are scratch.
+template <class Place, class Entry>
+void xt_walk_rows(const int* rp, const int* ci, int r0, int r1, std::vector<int>& rowcnt, std::vector<int>& touched,
+                  Place place, Entry entry) {
+    for (int r = r0; r < r1; ++r) {
+        for (long long k = rp[r]; k < rp[r + 1]; ++k) {
+            const int sl = ci[k] / kXtCols;
+            if (rowcnt[(size_t)sl]++ == 0) touched.push_back(sl);
+        }
+        for (int sl : touched) {
+            place(r, sl, (long long)rowcnt[(size_t)sl]);
+            rowcnt[(size_t)sl] = 0;
+        }
+        touched.clear();
+        for (long long k = rp[r]; k < rp[r + 1]; ++k) entry(r, k, ci[k] / kXtCols);
+    }
+}
+
 // Column groups per row block at most (tunable: "xtile_max_groups", 1..8).
 int g_xt_max_groups = kXtMaxGroups;
 
@@ -742,7 +777,11 @@ int g_xt_max_groups = kXtMaxGroups;
 // has about W workgroups.  When balancing by entries would need more
 // blocks than that minimum and so lower G, the rows are cut evenly instead.
 // Within a block the entries are bucketed by column slice in row order, so
-// each tile is sorted by (row, col) when the rows are.
+// each tile is sorted by (row, col) when the rows are; short rows are placed
+// so that they are not split across lanes or chunks (xt_place).  Each chunk
+// is flagged for the kernel's fast path when no row crosses one of its lane
+// boundaries, and with the carry bit when its first row continues the
+// previous chunk of the tile; a tile with a carry is flagged too.
 // `codec` (0 or a kCodec* value) encodes the values: FP16 divided by a
 // power-of-two scale per row, int16 fixed point with one scale per row, FP8
 // with one scale per 128-entry chunk.
@@ -779,16 +818,21 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
     }
     const int nb = (int)brow.size() - 1;
     // per block: (slice, entries) of its nonempty slices
-    std::vector<std::vector<std::pair<int, int>>> bt((size_t)nb);
+    std::vector<std::vector<std::pair<int, long long>>> bt((size_t)nb);
 #pragma omp parallel num_threads(omp_threads())
     {
-        std::vector<int> cnt((size_t)ns);
+        std::vector<long long> cnt((size_t)ns);
+        std::vector<int> rowcnt((size_t)ns, 0), touched;
 #pragma omp for schedule(dynamic, 4)
         for (int b = 0; b < nb; ++b) {
             std::fill(cnt.begin(), cnt.end(), 0);
-            for (long long k = rp[brow[b]]; k < rp[brow[b + 1]]; ++k) cnt[(size_t)(ci[k] / kXtCols)]++;
-            for (int s = 0; s < ns; ++s)
-                if (cnt[(size_t)s]) bt[(size_t)b].push_back({s, cnt[(size_t)s]});
+            // slots per slice, padding included
+            xt_walk_rows(
+                rp, ci, brow[b], brow[b + 1], rowcnt, touched,
+                [&](int, int sl, long long L) { cnt[(size_t)sl] = xt_place(cnt[(size_t)sl], L) + L; },
+                [](int, long long, int) {});
+            for (int sl = 0; sl < ns; ++sl)
+                if (cnt[(size_t)sl]) bt[(size_t)b].push_back({sl, cnt[(size_t)sl]});
         }
     }
     // Column groups: G contiguous ranges of slices; workgroup
@@ -805,7 +849,7 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
                 const long long nc = (t.second + kXtChunk - 1) / kXtChunk;
                 if (((long long)tile_chunk.back() + nc) * kXtChunk > INT32_MAX) return CLSPMV_E_TOO_LARGE;
                 for (long long k = 0; k < nc; k += kXtTileChunks) {
-                    tile_slice.push_back((t.first << 1) | (k ? 1 : 0));
+                    tile_slice.push_back((t.first << 2) | (k ? 1 : 0));
                     tile_chunk.push_back(tile_chunk.back() + (int)std::min<long long>(kXtTileChunks, nc - k));
                 }
             }
@@ -830,20 +874,47 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
 #pragma omp parallel num_threads(omp_threads())
     {
         std::vector<long long> pos((size_t)ns, -1);
+        std::vector<int> rowcnt((size_t)ns, 0), touched;
 #pragma omp for schedule(dynamic, 4)
         for (int b = 0; b < nb; ++b) {
             // entries of a slice run contiguously over its pieces
             for (int t = blk_tile[(size_t)b * G]; t < blk_tile[(size_t)(b + 1) * G]; ++t)
                 if (!(tile_slice[(size_t)t] & 1))
-                    pos[(size_t)(tile_slice[(size_t)t] >> 1)] = (long long)tile_chunk[(size_t)t] * kXtChunk;
-            for (int r = brow[b]; r < brow[b + 1]; ++r)
-                for (long long k = rp[r]; k < rp[r + 1]; ++k) {
-                    const int s = ci[k] / kXtCols;
-                    const long long q = pos[(size_t)s]++;
-                    idx[(size_t)q] = ((unsigned)(r - brow[b]) << 16) | (unsigned)(ci[k] - s * kXtCols);
+                    pos[(size_t)(tile_slice[(size_t)t] >> 2)] = (long long)tile_chunk[(size_t)t] * kXtChunk;
+            xt_walk_rows(
+                rp, ci, brow[b], brow[b + 1], rowcnt, touched,
+                [&](int, int sl, long long L) { pos[(size_t)sl] = xt_place(pos[(size_t)sl], L); },
+                [&](int r, long long k, int sl) {
+                    const long long q = pos[(size_t)sl]++;
+                    idx[(size_t)q] = ((unsigned)(r - brow[b]) << 16) | (unsigned)(ci[k] - sl * kXtCols);
                     val[(size_t)q] = rscale.empty() ? cv[k] : cv[k] / rscale[(size_t)r];
-                }
+                });
         }
+    }
+    // Chunk flags (k_xtile.hip), in the column half of the chunk's first
+    // entry: the carry bit when the chunk's first row continues the previous
+    // chunk of the tile (a padding slot never does), else the fast bit when
+    // no row crosses a lane boundary (entries 2l - 1 and 2l in different
+    // rows, or padding); a tile holding a carry gets bit 1 of its slice word.
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+    for (long long t = 0; t < (long long)tile_slice.size(); ++t) {
+        bool carries = false;
+        for (int c = tile_chunk[(size_t)t]; c < tile_chunk[(size_t)t + 1]; ++c) {
+            const size_t q = (size_t)c * kXtChunk;
+            const unsigned r0 = idx[q] >> 16;
+            if (c > tile_chunk[(size_t)t] && r0 == (idx[q - 1] >> 16)) {
+                idx[q] |= kXtCarryBit;
+                carries = true;
+                continue;
+            }
+            bool fast = true;
+            for (int l = 1; l < kXtChunk / 2 && fast; ++l) {
+                const unsigned h = idx[q + 2 * (size_t)l] >> 16;
+                fast = h == kXtPadRow || h != (idx[q + 2 * (size_t)l - 1] >> 16);
+            }
+            if (fast) idx[q] |= kXtFastBit;
+        }
+        if (carries) tile_slice[(size_t)t] |= 2;
     }
     int rc = 0;
     DevXtile d{};
