@@ -127,6 +127,17 @@ __device__ __forceinline__ void seg_step(bool valid, float up_v, int up_f, float
     f = take ? up_f : f;
 }
 
+// One step of a segmented inclusive scan over lanes whose keys are sorted:
+// a lane adds the partial sum of its DPP source lane when that lane has the
+// same key (then so has every lane between).  Sources outside the written
+// rows, or outside the wave, read key 0 (no lane's).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void xt_scan_step(int key, float& c) {
+    const int k = dpp_i<CTRL, ROWS>(key);
+    const float u = dpp_f<CTRL, ROWS>(c);
+    c = k == key ? c + u : c;
+}
+
 static_assert(kXtRounds * kXtWaves == 32, "the carry slots of a tile fill lanes 0..31 of one wave");
 
 // Adds the carry sets of the chunk rounds in `pending` (bit k: round k) to
@@ -156,10 +167,11 @@ __device__ __forceinline__ void xt_apply_carries(int pending, int (*c_row)[kXtWa
 
 // One wavefront reduces one 128-entry chunk (2 entries per lane).  Each
 // segment of equal rows is emitted once, by the lane holding its last
-// entry: at most two emit sites per lane in the common case, three in the
-// general one, all evaluated without branches.  The chunk's first row goes
-// to the chunk's carry slot when its first entry has the carry bit (the row
-// continues the previous chunk of the tile), to y_l otherwise.
+// entry: at most two emit sites per lane (the lane's head row when its two
+// rows differ, its tail row when the next lane does not go on with it), all
+// evaluated without branches.  The chunk's first row goes to the chunk's
+// carry slot when its first entry has the carry bit (the row continues the
+// previous chunk of the tile), to y_l otherwise.
 __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, float* y_l, float* dummy, int* cr,
                                          float* cv, int wave, int lane) {
     // products rounded, then added: no multiply-add contraction, so that
@@ -193,44 +205,33 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
     if (lane == 0) cr[wave] = carry ? first_row : -1;
     float* slot = &cv[wave];
 
-    const int prev_tail_row = lane_up1(tail_row);
-    const float prev_tail_sum = lane_up1(tail_sum);
-    const bool cont = lane > 0 && prev_tail_row == head_row;  // head continues the previous lane's tail
-    const bool through = cont && !brk;                        // the whole lane is inside that segment
-    const unsigned long long tmask = __ballot(through);
-    const unsigned long long cmask = __ballot(cont);
-    // A segment spans three or more lanes exactly when some lane L lies
-    // inside it (through) and lane L + 1 continues it.
-    if ((tmask & (cmask >> 1)) == 0) {
-        // Every segment spans at most two lanes: one neighbour exchange
-        // completes it (the common case for short rows).
-        const bool next_cont = lane < 63 && ((cmask >> (lane + 1)) & 1ull);
-        const bool on[2] = {brk, !next_cont};
-        const int row[2] = {head_row, tail_row};
-        const float sum[2] = {head_sum + (cont ? prev_tail_sum : 0.f), tail_sum + (through ? prev_tail_sum : 0.f)};
-        xt_emit<2>(on, row, sum, first_row, carry, y_l, dummy, slot, lane);
-    } else {
-        // General case: segmented inclusive scan of the lane tails, all in
-        // DPP moves: steps of 1, 2, 4, 8 lanes within each 16-lane row, then
-        // row 0's prefix into rows 1 and 3 (lane 15 of the row below) and
-        // rows 0-1's into rows 2 and 3 (lane 31).
-        int flag = (lane == 0) || brk || (prev_tail_row != tail_row);
-        float cval = tail_sum;
-        const int rl = lane & 15;
-        seg_step(rl >= 1, dpp_f<0x111, 0xf>(cval), dpp_i<0x111, 0xf>(flag), cval, flag);
-        seg_step(rl >= 2, dpp_f<0x112, 0xf>(cval), dpp_i<0x112, 0xf>(flag), cval, flag);
-        seg_step(rl >= 4, dpp_f<0x114, 0xf>(cval), dpp_i<0x114, 0xf>(flag), cval, flag);
-        seg_step(rl >= 8, dpp_f<0x118, 0xf>(cval), dpp_i<0x118, 0xf>(flag), cval, flag);
-        seg_step((lane & 16) != 0, dpp_f<0x142, 0xa>(cval), dpp_i<0x142, 0xa>(flag), cval, flag);
-        seg_step(lane >= 32, dpp_f<0x143, 0xc>(cval), dpp_i<0x143, 0xc>(flag), cval, flag);
-        const float prev_c = lane_up1(cval);
-        // a segment that ended exactly before this lane
-        const bool boundary_end = lane > 0 && prev_tail_row != head_row;
-        const bool on[3] = {boundary_end, brk, lane == 63};
-        const int row[3] = {prev_tail_row, head_row, tail_row};
-        const float sum[3] = {prev_c, head_sum + ((lane > 0 && !boundary_end) ? prev_c : 0.f), cval};
-        xt_emit<3>(on, row, sum, first_row, carry, y_l, dummy, slot, lane);
-    }
+    // Segmented inclusive scan of the lane tails (row b's sums) in DPP
+    // moves, as far as the chunk's scan plan (from the layout) asks: shifts
+    // of 1 .. 2^(steps-1) lanes within each 16-lane row cover every run of
+    // equal tail rows there, then lane 15 (47) into the next row and lane 31
+    // into rows 2 and 3 when a run crosses those lanes.
+    const int plan = __builtin_amdgcn_readfirstlane((int)e.y);
+    const int steps = ((plan >> 13) & 3) + 1;
+    const int key = tail_row + 1;  // lanes with no source read 0: never a row
+    float c = tail_sum;
+    xt_scan_step<0x111, 0xf>(key, c);
+    if (steps >= 2) xt_scan_step<0x112, 0xf>(key, c);
+    if (steps >= 3) xt_scan_step<0x114, 0xf>(key, c);
+    if (steps >= 4) xt_scan_step<0x118, 0xf>(key, c);
+    if (plan & (int)kXtBc15Bit) xt_scan_step<0x142, 0xa>(key, c);
+    if (flags & (int)kXtBc31Bit) xt_scan_step<0x143, 0xc>(key, c);
+    // a lane's head row continues the previous lane's tail row; its tail row
+    // ends here when the next lane's head row is another (past lane 63: -1).
+    // The lane moves are made with every lane active: a DPP read from a lane
+    // that is off in EXEC does not return that lane's value.
+    const int prev_tail = lane_up1(tail_row);
+    const int next_head = __builtin_amdgcn_update_dpp(-1, head_row, 0x130, 0xf, 0xf, false);
+    const float prev_c = lane_up1(c);
+    const bool cont = (lane > 0) & (prev_tail == head_row);
+    const bool on[2] = {brk, next_head != tail_row};
+    const int row[2] = {head_row, tail_row};
+    const float sum[2] = {head_sum + (cont ? prev_c : 0.f), c};
+    xt_emit<2>(on, row, sum, first_row, carry, y_l, dummy, slot, lane);
 }
 
 // Per-lane value storage of one chunk (two entries) for each codec.

@@ -117,9 +117,12 @@ constexpr bool codec_row_scaled(int c) { return c == kCodecFp16Row || c == kCode
 // LDS x-tiled layout (k_xtile.hip): row blocks of <= kXtRows rows, column
 // slices of kXtCols columns, entries packed (row_in_block << 16 |
 // col_in_slice) in chunks of kXtChunk; padding entries carry row kXtPadRow.
-// The column half of a chunk's first entry carries the chunk's flags:
-// kXtCarryBit when its first row continues the previous chunk of the same
-// tile, kXtFastBit when no row crosses one of its lane boundaries.
+// The column halves of a chunk's first two entries carry the chunk's flags:
+// entry 0 kXtCarryBit when its first row continues the previous chunk of
+// the same tile, kXtFastBit when no row crosses one of its lane boundaries,
+// and the scan plan of the other chunks -- entry 1 bits 13-14: in-row steps
+// - 1, entry 1 kXtBc15Bit / entry 0 kXtBc31Bit: a row continues from lane 15
+// into 16 or 47 into 48 / from lane 31 into 32 (k_xtile.hip).
 constexpr int kXtThreads = 1024;
 constexpr int kXtWaves = kXtThreads / 64;
 // LDS: y partial sums of kXtRows rows (128 KiB) + one x slice of kXtCols
@@ -134,9 +137,12 @@ constexpr int kXtRounds = 2;                       // chunk rounds per tile held
 constexpr int kXtTileChunks = kXtRounds * kXtWaves;  // a slice with more chunks is split
 constexpr int kXtDepth = 3;                        // tiles prefetched ahead
 constexpr unsigned kXtPadRow = 0xFFFFu;
-constexpr unsigned kXtCarryBit = 0x8000u;
-constexpr unsigned kXtFastBit = 0x4000u;
-constexpr unsigned kXtColMask = 0x3FFFu;
+constexpr unsigned kXtCarryBit = 0x8000u;  // entry 0
+constexpr unsigned kXtFastBit = 0x4000u;   // entry 0
+constexpr unsigned kXtBc31Bit = 0x2000u;   // entry 0
+constexpr unsigned kXtBc15Bit = 0x8000u;   // entry 1
+constexpr int kXtStepsShift = 13;          // entry 1, 2 bits
+constexpr unsigned kXtColMask = 0x1FFFu;
 static_assert(kXtCols <= (int)kXtColMask + 1, "packed entry fields");
 constexpr int kXtMaxGroups = 8;    // column groups per row block at most
 struct DevXtile {

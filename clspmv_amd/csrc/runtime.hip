@@ -731,39 +731,49 @@ int build_csr_sliced(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int
 
 int g_xt_blocks = 256;  // target workgroups: one per CU (tunable: "xtile_blocks")
 
-// Where a row's L entries in one column slice start, given the slice's next
-// free position p (slices start on chunk boundaries, so p mod kXtChunk is
-// the offset in the chunk): a row of 2 entries starts at an even offset, so
-// that one lane holds both, and a row of 3 to kXtAlignMax entries that would
-// straddle a chunk boundary starts the next chunk, so that its sum needs no
-// carry.  The positions skipped are padding.
+// A row's entries in one column slice (a segment of L entries) are placed
+// by class: 0 (L <= 2) first, in row order, then 1 (3 <= L <= kXtAlignMax),
+// then 2 (longer), so that chunks of short rows take the kernel's fast path
+// and the others need few scan steps.  Where a segment starts, given its
+// slice's next free position p (slices start on chunk boundaries, so p mod
+// kXtChunk is the offset in the chunk): segments of 2 to kXtAlignMax
+// entries start at an even offset (a row of 2 entries in one lane), and
+// class 1 does not straddle a chunk boundary (its sum needs no carry).  The
+// positions skipped are padding.
 constexpr int kXtAlignMax = 16;
+inline int xt_class(long long L) { return L <= 2 ? 0 : (L <= kXtAlignMax ? 1 : 2); }
 inline long long xt_place(long long p, long long L) {
+    const int cls = xt_class(L);
+    if (cls == 2 || L == 1) return p;
+    p += p & 1;
     const long long off = p % kXtChunk;
-    if (L == 2) return p + (off & 1);
-    if (L > 2 && L <= kXtAlignMax && off + L > kXtChunk) return p + (kXtChunk - off);
+    if (cls == 1 && off + L > kXtChunk) p += kXtChunk - off;
     return p;
 }
 
-// Visits the rows r0 .. r1 - 1 in order; for each row, first calls
-// place(slice, L) once for every column slice in which the row has L > 0
-// entries (in order of first appearance), then entry(k, slice) for each of
-// its entries.  rowcnt (ns zeros) and touched are scratch.
+// Visits the rows r0 .. r1 - 1 in order, three times (segment classes 0, 1,
+// 2); in pass c, for each row, first calls place(row, slice, L) for every
+// column slice in which the row has a segment of class c (L entries), then
+// entry(row, k, slice) for each entry of those segments.  rowcnt (ns zeros)
+// and touched are scratch.
 template <class Place, class Entry>
 void xt_walk_rows(const int* rp, const int* ci, int r0, int r1, std::vector<int>& rowcnt, std::vector<int>& touched,
                   Place place, Entry entry) {
-    for (int r = r0; r < r1; ++r) {
-        for (long long k = rp[r]; k < rp[r + 1]; ++k) {
-            const int sl = ci[k] / kXtCols;
-            if (rowcnt[(size_t)sl]++ == 0) touched.push_back(sl);
+    for (int cls = 0; cls < 3; ++cls)
+        for (int r = r0; r < r1; ++r) {
+            for (long long k = rp[r]; k < rp[r + 1]; ++k) {
+                const int sl = ci[k] / kXtCols;
+                if (rowcnt[(size_t)sl]++ == 0) touched.push_back(sl);
+            }
+            for (int sl : touched)
+                if (xt_class(rowcnt[(size_t)sl]) == cls) place(r, sl, (long long)rowcnt[(size_t)sl]);
+            for (long long k = rp[r]; k < rp[r + 1]; ++k) {
+                const int sl = ci[k] / kXtCols;
+                if (xt_class(rowcnt[(size_t)sl]) == cls) entry(r, k, sl);
+            }
+            for (int sl : touched) rowcnt[(size_t)sl] = 0;
+            touched.clear();
         }
-        for (int sl : touched) {
-            place(r, sl, (long long)rowcnt[(size_t)sl]);
-            rowcnt[(size_t)sl] = 0;
-        }
-        touched.clear();
-        for (long long k = rp[r]; k < rp[r + 1]; ++k) entry(r, k, ci[k] / kXtCols);
-    }
 }
 
 // Column groups per row block at most (tunable: "xtile_max_groups", 1..8).
@@ -776,12 +786,13 @@ int g_xt_max_groups = kXtMaxGroups;
 // split into G = W / blocks (<= 8) column groups so that the launch still
 // has about W workgroups.  When balancing by entries would need more
 // blocks than that minimum and so lower G, the rows are cut evenly instead.
-// Within a block the entries are bucketed by column slice in row order, so
-// each tile is sorted by (row, col) when the rows are; short rows are placed
-// so that they are not split across lanes or chunks (xt_place).  Each chunk
-// is flagged for the kernel's fast path when no row crosses one of its lane
-// boundaries, and with the carry bit when its first row continues the
-// previous chunk of the tile; a tile with a carry is flagged too.
+// Within a block the entries are bucketed by column slice, each row's
+// entries of a slice contiguous (in column order when the row's are), the
+// rows ordered by segment class and then by row (xt_place).  Each chunk is
+// flagged for the kernel's fast path when no row crosses one of its lane
+// boundaries, with the carry bit when its first row continues the previous
+// chunk of the tile (a tile with a carry is flagged too), and otherwise
+// carries the plan of its segmented scan.
 // `codec` (0 or a kCodec* value) encodes the values: FP16 divided by a
 // power-of-two scale per row, int16 fixed point with one scale per row, FP8
 // with one scale per 128-entry chunk.
@@ -891,28 +902,43 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
                 });
         }
     }
-    // Chunk flags (k_xtile.hip), in the column half of the chunk's first
-    // entry: the carry bit when the chunk's first row continues the previous
-    // chunk of the tile (a padding slot never does), else the fast bit when
-    // no row crosses a lane boundary (entries 2l - 1 and 2l in different
-    // rows, or padding); a tile holding a carry gets bit 1 of its slice word.
+    // Chunk flags (k_xtile.hip), in the column halves of the chunk's first
+    // two entries: the carry bit when the chunk's first row continues the
+    // previous chunk of the tile (a padding slot never does); the fast bit
+    // when there is no carry and no row crosses a lane boundary (entries
+    // 2l - 1 and 2l in different rows, or padding); otherwise the scan plan
+    // over the lanes' tail rows (entries 2l + 1, padding never matching):
+    // in-row steps = ceil(log2 of the longest run of equal tail rows within
+    // a 16-lane row), at least 1, and whether a run crosses lanes 15|16 or
+    // 47|48 (bc15) and 31|32 (bc31).  A tile holding a carry gets bit 1 of
+    // its slice word.
 #pragma omp parallel for schedule(static) num_threads(omp_threads())
     for (long long t = 0; t < (long long)tile_slice.size(); ++t) {
         bool carries = false;
         for (int c = tile_chunk[(size_t)t]; c < tile_chunk[(size_t)t + 1]; ++c) {
             const size_t q = (size_t)c * kXtChunk;
-            const unsigned r0 = idx[q] >> 16;
-            if (c > tile_chunk[(size_t)t] && r0 == (idx[q - 1] >> 16)) {
+            auto row = [&](int j) { return idx[q + (size_t)j] >> 16; };
+            const bool carry = c > tile_chunk[(size_t)t] && row(0) == (idx[q - 1] >> 16);
+            bool fast = !carry;
+            for (int l = 1; l < kXtChunk / 2 && fast; ++l) fast = row(2 * l) == kXtPadRow || row(2 * l) != row(2 * l - 1);
+            if (carry) {
                 idx[q] |= kXtCarryBit;
                 carries = true;
+            } else if (fast) {
+                idx[q] |= kXtFastBit;
                 continue;
             }
-            bool fast = true;
-            for (int l = 1; l < kXtChunk / 2 && fast; ++l) {
-                const unsigned h = idx[q + 2 * (size_t)l] >> 16;
-                fast = h == kXtPadRow || h != (idx[q + 2 * (size_t)l - 1] >> 16);
+            auto same_tail = [&](int l) { return row(2 * l + 1) != kXtPadRow && row(2 * l + 1) == row(2 * l - 1); };
+            int run = 1, longest = 1;
+            for (int l = 1; l < kXtChunk / 2; ++l) {
+                run = (l % 16 != 0 && same_tail(l)) ? run + 1 : 1;
+                longest = std::max(longest, run);
             }
-            if (fast) idx[q] |= kXtFastBit;
+            int steps = 1;
+            while ((1 << steps) < longest) ++steps;
+            idx[q + 1] |= (unsigned)(steps - 1) << kXtStepsShift;
+            if (same_tail(16) || same_tail(48)) idx[q + 1] |= kXtBc15Bit;
+            if (same_tail(32)) idx[q] |= kXtBc31Bit;
         }
         if (carries) tile_slice[(size_t)t] |= 2;
     }
