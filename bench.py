@@ -712,6 +712,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the Cocktail composite leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-launch", action="store_true",
+                    help="skip the launch-overhead leg (HIP graph capture), e.g. under a kernel tracer")
     ap.add_argument("--c4-matrices", type=int, default=1024,
                     help="matrices of the batched leg (default 1024, the c4 set; 0 skips the leg)")
     args = ap.parse_args()
@@ -848,7 +850,7 @@ def main():
     c3 = None if args.no_c3 else mixed_c3(ctx, args, work, per)
     c2 = None if args.no_c2 else cocktail_c2(ctx, args)
     c4 = batched_c4(ctx, args) if args.c4_matrices > 0 else None
-    launch = launch_c0(ctx)
+    launch = None if args.no_launch else launch_c0(ctx)
 
     cpu = None
     if not args.no_cpu:

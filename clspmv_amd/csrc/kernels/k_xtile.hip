@@ -15,17 +15,23 @@
 //     tile loop is unrolled by kXtDepth so that each set stays in place),
 //     which covers the HBM/MALL latency that a one-tile prefetch leaves
 //     exposed;
-//   * a tile's entries are sorted by (row, col), packed as
-//     (row_in_block << 16 | col_in_slice) plus the value (FP32: a lane's two
-//     entries and two values in one 16-byte word), and padded to 128-entry
-//     chunks.  A wavefront reduces one chunk (2 entries per lane) with a
-//     segmented scan whose lane exchanges are all DPP moves (no trip through
-//     the LDS crossbar); a segment that starts inside the chunk is added to
-//     its row by that wavefront alone, the chunk's first segment (which may
-//     continue an earlier chunk) goes to a per-chunk carry slot.  A tile's
-//     up to kXtRounds rounds of chunks run back to back; after the next
-//     tile's first barrier one wave sums the carry slots of all its rounds
-//     per run of equal rows (another DPP scan) and adds each run once.
+//   * a tile's entries are grouped by row (each row's entries of the slice
+//     contiguous; rows of 1-2 entries first, then of 3-16, then longer, so
+//     that chunks are alike), packed as (row_in_block << 16 | col_in_slice)
+//     plus the value (FP32: a lane's two entries and two values in one
+//     16-byte word), and padded to 128-entry chunks; rows of 2-16 entries
+//     start on a lane and rows of 3-16 do not cross a chunk boundary.  A
+//     wavefront reduces one chunk (2 entries per lane).  A chunk in which no
+//     row crosses a lane boundary (flagged by the layout) adds each lane's
+//     sums to y_l directly; the others run a segmented scan over the lane
+//     tails whose lane exchanges are all DPP moves (no trip through the LDS
+//     crossbar), with only the steps the chunk's plan (also from the
+//     layout) needs.  Each row's sum is added by one lane, unless the row
+//     continues the previous chunk of the tile (the chunk's carry bit): that
+//     partial sum goes to the chunk's carry slot.  A tile's up to kXtRounds
+//     rounds of chunks run back to back; after the next tile's first
+//     barrier one wave sums the carry slots of a tile that has any, per run
+//     of equal rows (another DPP scan), and adds each run once.
 //   * x is staged once per row block (the x traffic is row blocks x x
 //     bytes), so row blocks are as tall as LDS allows; to still give every
 //     CU a workgroup, the slices are split into G <= 8 contiguous column
