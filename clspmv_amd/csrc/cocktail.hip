@@ -449,10 +449,12 @@ signed char classify_block(const clspmv_csr_matrix& m, const std::vector<int>& r
 // Layout (format, method) of a row run of class cls with `rows` rows and
 // n entries.  Sizes fitted on MI355X against the autotuner's timings of
 // 2400 matrices of 2^10..2^20 rows:
-//  * scattered rows: x-tiled (method 4) once the run has >= 2^20.3 entries
-//    and >= 6 per row, also with x inside L2 (Zipf rows: 1.2-1.6x faster
-//    than adaptive row blocks above that, slower below); with x wider than
-//    L2 by the cost rule (x-tiled or column-sliced); else adaptive rows;
+//  * scattered rows: x-tiled (method 4) once the run has >= 2^14 entries
+//    and >= 5 per row, or >= 2^19 entries, also with x inside L2 (400 Zipf
+//    matrices: 2.0 % mean regret between the two layouts, against 7.5 %
+//    for the rule fitted to the x-tiled kernel before its chunk fast path);
+//    with x wider than L2 by the cost rule (x-tiled or column-sliced); else
+//    adaptive rows;
 //  * near-equal scattered rows: SELL, x-tiled from 2^21.5 entries and >= 12
 //    per row (uniform rows: 1.1-1.8x faster there);
 //  * DIA: one row per lane from 1.5M entries, below that 2-8 lanes per row
@@ -470,7 +472,7 @@ void run_layout(signed char cls, int rows, int cols, long long n, int* fmt, int*
     switch (cls) {
         case kScattered:
             if (wide_x(cols)) *meth = xtile_pays(rows, cols, n) ? kXtile : 2;
-            else if (n >= (1LL << 20) * 5 / 4 && mean >= 6.0) *meth = kXtile;
+            else if ((n >= (1LL << 14) && mean >= 5.0) || n >= (1LL << 19)) *meth = kXtile;
             break;
         case kScatteredSell:
             if (n >= 3LL << 20 && mean >= 12.0) *meth = kXtile;
