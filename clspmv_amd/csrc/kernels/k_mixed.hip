@@ -223,6 +223,16 @@ __global__ __launch_bounds__(256) void dia_mixed_kernel(const int* __restrict__ 
     }
 }
 
+// Position of value e of the block in slot idx (slots come in steps of 64,
+// one per lane of a slice's wave): the BS values of a step are stored one
+// after another, 64 of each, so that a wave's value loads of one step read
+// one contiguous run of 64 x BS elements rather than BS streams far apart
+// (HBM reads such streams ~20 % slower).
+template <int BS>
+__device__ __forceinline__ size_t bsr_vpos(int idx, int e) {
+    return ((size_t)(idx >> 6) * BS + e) * 64 + (idx & 63);
+}
+
 // Value e of the block in slot idx, decoded (FP8: times the scale of element
 // e in slice s; FP16, int16: the row scale is applied by the caller).
 template <int CODEC>
@@ -282,7 +292,7 @@ __global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_mixed_kernel(
         for (int u = 0; u < G; ++u)
 #pragma unroll
             for (int e = 0; e < BH * BW; ++e)
-                v[u][e] = bsr_value<CODEC>(data, scale, (size_t)e * stride + b + ((k + u) << 6) + t, e, nsl, s);
+                v[u][e] = bsr_value<CODEC>(data, scale, bsr_vpos<BH * BW>(b + ((k + u) << 6) + t, e), e, nsl, s);
     };
     auto do_group = [&](int (&c)[G], float (&v)[G][BH * BW]) {
         if (DELTA) {
@@ -337,7 +347,7 @@ __global__ __launch_bounds__(XW ? kBsrWinRows : 256) void bsr_mixed_kernel(
         for (int h = 0; h < BH; ++h)
 #pragma unroll
             for (int w = 0; w < BW; ++w)
-                acc[h] = fmaf(bsr_value<CODEC>(data, scale, (size_t)(h * BW + w) * stride + idx, h * BW + w, nsl, s),
+                acc[h] = fmaf(bsr_value<CODEC>(data, scale, bsr_vpos<BH * BW>(idx, h * BW + w), h * BW + w, nsl, s),
                               xv[w], acc[h]);
     }
     const float rs = codec_row_scaled(CODEC) ? scale[r] : 1.f;

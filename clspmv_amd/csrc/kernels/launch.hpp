@@ -172,7 +172,7 @@ struct DevBsrMixed {
     const int* sp;        // slice pointer of the sliced layout
     const void* col;      // int32 element columns, or int16 deltas
     const int* base;      // per block row: column the first delta is added to
-    const void* data;     // bh*bw * stride encoded values
+    const void* data;     // bh*bw * stride encoded values, per 64-slot step the bh*bw elements' 64 values in turn
     const float* scale;   // FP16: one per block row; FP8: bh*bw * nsl
     int stride, brows, nsl, bh, bw, rows, cols, codec, delta;
     const int2* win;      // per-workgroup x windows (LDS-windowed kernel), or null

@@ -1162,6 +1162,18 @@ int build_bsr_coded(clspmv_ctx_t ctx, clspmv_part_t p, const clspmv_bsr_matrix* 
                     }
                 }
         }
+        // device order of the values: per 64-slot step of a slice, the bs
+        // elements' 64 values one after another (k_mixed.hip: bsr_vpos)
+        {
+            const size_t esz = enc.size() / nv;
+            std::vector<unsigned char> step(enc.size(), 0);
+#pragma omp parallel for schedule(static) num_threads(omp_threads())
+            for (long long i = 0; i < (long long)slots; ++i)
+                for (unsigned q = 0; q < bs; ++q)
+                    std::memcpy(&step[(((size_t)i >> 6) * bs + q) * 64 * esz + ((size_t)i & 63) * esz],
+                                &enc[((size_t)q * sstride + (size_t)i) * esz], esz);
+            enc.swap(step);
+        }
         int rc = 0;
         DevBsrMixed d{};
         d.sp = (const int*)p->add(ctx, sp.data(), sp.size() * 4, &rc);
