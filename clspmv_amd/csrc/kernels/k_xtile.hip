@@ -306,6 +306,8 @@ __device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int
     S.c1 = c1;
 #pragma unroll
     for (int k = 0; k < kXtRounds; ++k) {
+        // rounds past the tile's chunks load nothing (workgroup-uniform)
+        if (k > 0 && c1 - c0 <= k * kXtWaves) break;
         // wave-uniform chunk index (the FP8 scale becomes a scalar load);
         // a wave with no chunk in this chunk round reloads the tile's last chunk
         // (a cache hit) rather than fetching the next tile's entries twice
