@@ -739,12 +739,15 @@ int g_xt_blocks = 256;  // target workgroups: one per CU (tunable: "xtile_blocks
 // kXtChunk is the offset in the chunk): segments of 2 to kXtAlignMax
 // entries start at an even offset (a row of 2 entries in one lane), and
 // class 1 does not straddle a chunk boundary (its sum needs no carry).  The
-// positions skipped are padding.
+// positions skipped are padding.  A slice is placed without this padding
+// (`aligned` false) when the padding would cost its tiles another round of
+// chunks (xt_rounds).
 constexpr int kXtAlignMax = 16;
 inline int xt_class(long long L) { return L <= 2 ? 0 : (L <= kXtAlignMax ? 1 : 2); }
-inline long long xt_place(long long p, long long L) {
+inline long long xt_rounds(long long slots) { return ((slots + kXtChunk - 1) / kXtChunk + kXtWaves - 1) / kXtWaves; }
+inline long long xt_place(long long p, long long L, bool aligned) {
     const int cls = xt_class(L);
-    if (cls == 2 || L == 1) return p;
+    if (!aligned || cls == 2 || L == 1) return p;
     p += p & 1;
     const long long off = p % kXtChunk;
     if (cls == 1 && off + L > kXtChunk) p += kXtChunk - off;
@@ -829,21 +832,33 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
     }
     const int nb = (int)brow.size() - 1;
     // per block: (slice, entries) of its nonempty slices
-    std::vector<std::vector<std::pair<int, long long>>> bt((size_t)nb);
+    struct XtSliceFill {
+        int slice;
+        long long slots;  // entries + padding
+        bool aligned;     // placed with the alignment padding
+    };
+    std::vector<std::vector<XtSliceFill>> bt((size_t)nb);
 #pragma omp parallel num_threads(omp_threads())
     {
-        std::vector<long long> cnt((size_t)ns);
+        std::vector<long long> cnt((size_t)ns), plain((size_t)ns);
         std::vector<int> rowcnt((size_t)ns, 0), touched;
 #pragma omp for schedule(dynamic, 4)
         for (int b = 0; b < nb; ++b) {
             std::fill(cnt.begin(), cnt.end(), 0);
-            // slots per slice, padding included
+            std::fill(plain.begin(), plain.end(), 0);
+            // slots per slice, with and without the alignment padding
             xt_walk_rows(
                 rp, ci, brow[b], brow[b + 1], rowcnt, touched,
-                [&](int, int sl, long long L) { cnt[(size_t)sl] = xt_place(cnt[(size_t)sl], L) + L; },
+                [&](int, int sl, long long L) {
+                    cnt[(size_t)sl] = xt_place(cnt[(size_t)sl], L, true) + L;
+                    plain[(size_t)sl] += L;
+                },
                 [](int, long long, int) {});
-            for (int sl = 0; sl < ns; ++sl)
-                if (cnt[(size_t)sl]) bt[(size_t)b].push_back({sl, cnt[(size_t)sl]});
+            for (int sl = 0; sl < ns; ++sl) {
+                if (!plain[(size_t)sl]) continue;
+                const bool aligned = xt_rounds(cnt[(size_t)sl]) <= xt_rounds(plain[(size_t)sl]);
+                bt[(size_t)b].push_back({sl, aligned ? cnt[(size_t)sl] : plain[(size_t)sl], aligned});
+            }
         }
     }
     // Column groups: G contiguous ranges of slices; workgroup
@@ -856,11 +871,11 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
     for (int b = 0; b < nb; ++b) {
         for (int g = 0; g < G; ++g) {
             for (auto& t : bt[(size_t)b]) {
-                if (t.first / spg != g) continue;
-                const long long nc = (t.second + kXtChunk - 1) / kXtChunk;
+                if (t.slice / spg != g) continue;
+                const long long nc = (t.slots + kXtChunk - 1) / kXtChunk;
                 if (((long long)tile_chunk.back() + nc) * kXtChunk > INT32_MAX) return CLSPMV_E_TOO_LARGE;
                 for (long long k = 0; k < nc; k += kXtTileChunks) {
-                    tile_slice.push_back((t.first << 2) | (k ? 1 : 0));
+                    tile_slice.push_back((t.slice << 2) | (k ? 1 : 0));
                     tile_chunk.push_back(tile_chunk.back() + (int)std::min<long long>(kXtTileChunks, nc - k));
                 }
             }
@@ -885,6 +900,7 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
 #pragma omp parallel num_threads(omp_threads())
     {
         std::vector<long long> pos((size_t)ns, -1);
+        std::vector<char> aligned((size_t)ns, 1);
         std::vector<int> rowcnt((size_t)ns, 0), touched;
 #pragma omp for schedule(dynamic, 4)
         for (int b = 0; b < nb; ++b) {
@@ -892,9 +908,10 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
             for (int t = blk_tile[(size_t)b * G]; t < blk_tile[(size_t)(b + 1) * G]; ++t)
                 if (!(tile_slice[(size_t)t] & 1))
                     pos[(size_t)(tile_slice[(size_t)t] >> 2)] = (long long)tile_chunk[(size_t)t] * kXtChunk;
+            for (auto& f : bt[(size_t)b]) aligned[(size_t)f.slice] = f.aligned;
             xt_walk_rows(
                 rp, ci, brow[b], brow[b + 1], rowcnt, touched,
-                [&](int, int sl, long long L) { pos[(size_t)sl] = xt_place(pos[(size_t)sl], L); },
+                [&](int, int sl, long long L) { pos[(size_t)sl] = xt_place(pos[(size_t)sl], L, aligned[(size_t)sl]); },
                 [&](int r, long long k, int sl) {
                     const long long q = pos[(size_t)sl]++;
                     idx[(size_t)q] = ((unsigned)(r - brow[b]) << 16) | (unsigned)(ci[k] - sl * kXtCols);
