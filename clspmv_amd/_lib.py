@@ -91,6 +91,14 @@ class PlanPart(C.Structure):
                 ("nnz", C.c_longlong), ("block", C.c_int)]
 
 
+class XtileLayout(C.Structure):
+    """clspmv_xtile_layout: host arrays of a CSR matrix's LDS x-tiled layout."""
+    _fields_ = [("nblocks", C.c_int), ("groups", C.c_int), ("ntiles", C.c_int), ("nchunks", C.c_int),
+                ("blk_row", C.POINTER(C.c_int)), ("blk_tile", C.POINTER(C.c_int)),
+                ("tile_slice", C.POINTER(C.c_int)), ("tile_chunk", C.POINTER(C.c_int)),
+                ("idx", C.POINTER(C.c_uint)), ("val", C.POINTER(C.c_float))]
+
+
 class BatchEntry(C.Structure):
     _fields_ = [("csr", C.c_void_p), ("bsr", C.c_void_p), ("k", C.c_int)]
 
@@ -206,6 +214,8 @@ _SIGS = [
     ("clspmv_plan_cocktail", C.c_int, [C.c_void_p, _P(C.c_void_p)]),
     ("clspmv_plan_num_parts", C.c_int, [C.c_void_p]),
     ("clspmv_plan_get_part", C.c_int, [C.c_void_p, C.c_int, _P(PlanPart)]),
+    ("clspmv_xtile_layout_csr", C.c_int, [_P(Csr), _P(XtileLayout)]),
+    ("clspmv_free_xtile_layout", None, [_P(XtileLayout)]),
     ("clspmv_plan_partition", C.c_void_p, [C.c_void_p]),
     ("clspmv_plan_free", C.c_int, [C.c_void_p]),
     ("clspmv_init_plan_kernels", C.c_int, [C.c_void_p, C.c_void_p, _FP, _FP, _P(C.c_void_p)]),

@@ -799,8 +799,17 @@ int g_xt_max_groups = kXtMaxGroups;
 // `codec` (0 or a kCodec* value) encodes the values: FP16 divided by a
 // power-of-two scale per row, int16 fixed point with one scale per row, FP8
 // with one scale per 128-entry chunk.
-int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
-                int cols, int codec) {
+// The x-tiled layout's host arrays (everything build_xtile uploads, before
+// the FP32 interleave and the value codecs' encoding; values divided by
+// the row scale for row-scaled codecs).
+struct XtHostLayout {
+    std::vector<int> brow, blk_tile, tile_slice, tile_chunk;
+    std::vector<unsigned> idx;
+    std::vector<float> val, rscale;
+    int nb = 0, G = 1;
+};
+
+int xt_host_layout(const int* rp, const int* ci, const float* cv, int rows, int cols, int codec, XtHostLayout* out) {
     const long long nnz = rows ? rp[rows] : 0;
     if (cols < 4) {
         set_error("the x-tiled layout needs at least 4 columns");
@@ -959,6 +968,26 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
         }
         if (carries) tile_slice[(size_t)t] |= 2;
     }
+    out->brow.swap(brow);
+    out->blk_tile.swap(blk_tile);
+    out->tile_slice.swap(tile_slice);
+    out->tile_chunk.swap(tile_chunk);
+    out->idx.swap(idx);
+    out->val.swap(val);
+    out->rscale.swap(rscale);
+    out->nb = nb;
+    out->G = G;
+    return CLSPMV_OK;
+}
+
+int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci, const float* cv, int rows,
+                int cols, int codec) {
+    XtHostLayout H;
+    if (int e = xt_host_layout(rp, ci, cv, rows, cols, codec, &H)) return e;
+    std::vector<int>&brow = H.brow, &blk_tile = H.blk_tile, &tile_slice = H.tile_slice, &tile_chunk = H.tile_chunk;
+    std::vector<unsigned>& idx = H.idx;
+    std::vector<float>&val = H.val, &rscale = H.rscale;
+    const int nb = H.nb, G = H.G;
     int rc = 0;
     DevXtile d{};
     d.blk_row = (const int*)p->add(ctx, brow.data(), brow.size() * 4, &rc);
@@ -1995,6 +2024,51 @@ int clspmv_probe_launch_grid(clspmv_ctx_t ctx, int groups, int threads, int ntim
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     *usec = ms * 1000.f / ntimes;
     return CLSPMV_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Host arrays of a CSR matrix's LDS x-tiled layout (xt_host_layout), for
+// tests and layout inspection; no device work.
+int clspmv_xtile_layout_csr(const clspmv_csr_matrix* m, clspmv_xtile_layout* out) {
+    if (!m || !out) return CLSPMV_E_INVALID;
+    *out = clspmv_xtile_layout{};
+    if (int rc = validate_csr(m)) return rc;
+    XtHostLayout H;
+    if (int rc = xt_host_layout(m->csr_row_ptr, m->csr_col_id, m->csr_data, m->matinfo.height, m->matinfo.width, 0,
+                                &H))
+        return rc;
+    auto copy = [](const auto& v, auto** dst) {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        *dst = (T*)std::malloc(std::max<size_t>(1, v.size()) * sizeof(T));
+        if (*dst && !v.empty()) std::memcpy(*dst, v.data(), v.size() * sizeof(T));
+        return *dst != nullptr;
+    };
+    out->nblocks = H.nb;
+    out->groups = H.G;
+    out->ntiles = (int)H.tile_slice.size();
+    out->nchunks = H.tile_chunk.back();
+    const bool ok = copy(H.brow, &out->blk_row) && copy(H.blk_tile, &out->blk_tile) &&
+                    copy(H.tile_slice, &out->tile_slice) && copy(H.tile_chunk, &out->tile_chunk) &&
+                    copy(H.idx, &out->idx) && copy(H.val, &out->val);
+    if (!ok) {
+        clspmv_free_xtile_layout(out);
+        return CLSPMV_E_NOMEM;
+    }
+    return CLSPMV_OK;
+}
+
+void clspmv_free_xtile_layout(clspmv_xtile_layout* l) {
+    if (!l) return;
+    std::free(l->blk_row);
+    std::free(l->blk_tile);
+    std::free(l->tile_slice);
+    std::free(l->tile_chunk);
+    std::free(l->idx);
+    std::free(l->val);
+    *l = clspmv_xtile_layout{};
 }
 
 }  // extern "C"

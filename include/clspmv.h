@@ -340,6 +340,36 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
  * slices, a block's slices split into up to 8 column groups (for wide x with
  * scattered columns; needs width >= 4).  The
  * reference's method ids (e.g. texture variants) map onto these. */
+/* Host arrays of the LDS x-tiled layout a CSR part builds (method 4), as
+ * they are uploaded, before the FP32 interleave: no counterpart in the
+ * reference; for tests and for callers that inspect the layout (no device
+ * work; the options xtile_blocks and xtile_max_groups apply).  Row block b
+ * holds rows blk_row[b] .. blk_row[b+1]-1; workgroup w = b * groups + g
+ * (column group g) holds tiles blk_tile[w] .. blk_tile[w+1]-1; tile t holds
+ * chunks tile_chunk[t] .. tile_chunk[t+1]-1 of 128 entries and has the slice
+ * word tile_slice[t] = (column slice << 2) | 2 if one of its chunks has the
+ * carry bit | 1 if it continues the previous tile's slice.  Entry j of the
+ * chunks: idx[j] = (row in block << 16) | column in the 8064-column slice
+ * (bits 0-12), val[j] its value; padding entries have row 0xFFFF and value
+ * 0.  Bits 13-15 of a chunk's first two entries hold its flags: entry 0 bit
+ * 15 carry (the chunk's first row continues the previous chunk of the
+ * tile), bit 14 fast (no row crosses a lane boundary: entries 2l-1 and 2l
+ * hold different rows or padding), bit 13 a row continues from lane 31 into
+ * 32; entry 1 bits 13-14 the in-row scan steps - 1, bit 15 a row continues
+ * from lane 15 into 16 or from 47 into 48 (lane l holds entries 2l, 2l+1).
+ * Free with clspmv_free_xtile_layout. */
+typedef struct {
+    int nblocks, groups, ntiles, nchunks;
+    int* blk_row;          /* nblocks + 1 */
+    int* blk_tile;         /* nblocks * groups + 1 */
+    int* tile_slice;       /* ntiles */
+    int* tile_chunk;       /* ntiles + 1 */
+    unsigned* idx;         /* nchunks * 128 */
+    float* val;            /* nchunks * 128 */
+} clspmv_xtile_layout;
+int clspmv_xtile_layout_csr(const clspmv_csr_matrix* m, clspmv_xtile_layout* out);
+void clspmv_free_xtile_layout(clspmv_xtile_layout* l);
+
 /* Mixed-precision parts (no counterpart in the reference; for callers that
  * trade value precision for bytes per nonzero): values stored as
  *   CLSPMV_VAL_BF16X2    hi + lo, two BF16 halves (4 B, ~16 significant bits)
