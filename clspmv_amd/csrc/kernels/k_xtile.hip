@@ -281,7 +281,7 @@ struct XtArgs {
     const int* tile_slice;  // (slice << 2) | 2 if the tile has a carry | 1 if it continues the previous slice
     const int* tile_chunk;
     const u32x2* idx2;
-    const void* val;      // chunk values, XtVal<CODEC>::T per lane
+    const void* val;      // FP8: chunk values, XtVal<CODEC>::T per lane (the other codecs' are in idx2's records)
     const float* vscale;  // FP16: row scales; FP8: chunk scales
     int cols;
     const float* x;
@@ -299,6 +299,9 @@ __device__ __forceinline__ void xt_fetch_x(XtSet<CODEC>& S, const XtArgs& a, int
     for (int q = 0; q < kXtXLoads; ++q)
         S.x[q] = load_x4(a.x, base + min(q * kXtThreads + tid, kXtX4 - 1) * 4, a.cols);
 }
+struct XtW3 {  // a lane's 12-byte record (loaded as one 12-byte access)
+    unsigned w[3];
+};
 template <int CODEC>
 __device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int c0, int c1, int wave, int lane) {
     typedef typename XtVal<CODEC>::T V;
@@ -312,12 +315,20 @@ __device__ __forceinline__ void xt_fetch_e(XtSet<CODEC>& S, const XtArgs& a, int
         // a wave with no chunk in this chunk round reloads the tile's last chunk
         // (a cache hit) rather than fetching the next tile's entries twice
         const int c = __builtin_amdgcn_readfirstlane(min(c0 + wave + k * kXtWaves, c1 - 1));
-        if constexpr (CODEC == 0) {
-            // FP32: a lane's two entries and two values interleaved, one
-            // 16-byte load (half the load instructions of two streams)
+        if constexpr (CODEC == 0 || CODEC == kCodecBf16x2) {
+            // FP32, BF16 hi+lo: a lane's two entries and two values in one
+            // 16-byte record, one load (half the load instructions of two
+            // streams)
             const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(a.idx2) + (size_t)c * 64 + lane);
             S.e[k] = u32x2{q.x, q.y};
-            S.v[k] = f32x2{__uint_as_float(q.z), __uint_as_float(q.w)};
+            if constexpr (CODEC == 0) S.v[k] = f32x2{__uint_as_float(q.z), __uint_as_float(q.w)};
+            else S.v[k] = u32x2{q.z, q.w};
+        } else if constexpr (codec_row_scaled(CODEC)) {
+            // FP16, int16: two entries and the two 2-byte values, 12 bytes
+            const XtW3* r = reinterpret_cast<const XtW3*>(a.idx2) + (size_t)c * 64 + lane;
+            const unsigned e0 = __builtin_nontemporal_load(&r->w[0]), e1 = __builtin_nontemporal_load(&r->w[1]);
+            S.e[k] = u32x2{e0, e1};
+            S.v[k] = __builtin_nontemporal_load(&r->w[2]);
         } else {
             S.e[k] = __builtin_nontemporal_load(a.idx2 + (size_t)c * 64 + lane);
             S.v[k] = __builtin_nontemporal_load(reinterpret_cast<const V*>(a.val) + (size_t)c * 64 + lane);

@@ -151,8 +151,9 @@ struct DevXtile {
     const int* tile_slice;  // (column slice << 2) | 2 if a chunk of the tile has the carry bit
                             //   | 1 if the tile continues the previous tile's slice
     const int* tile_chunk;  // ntiles + 1 chunk ranges, at most kXtTileChunks per tile
-    const unsigned* idx;
-    const void* val;        // FP32 values, or encoded with `codec` (k_xtile.hip)
+    const unsigned* idx;    // per chunk and lane: FP32, BF16 hi+lo {entry, entry, value, value};
+                            //   FP16, int16 {entry, entry, two 2-byte values}; FP8 {entry, entry}
+    const void* val;        // FP8 values (the other codecs: idx)
     const float* vscale;    // FP16: rows row scales; FP8: nchunks chunk scales
     float* partial;         // groups * rows partial sums when groups > 1
     int nblocks;            // workgroups = row blocks * groups

@@ -1005,35 +1005,35 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
         p->xt_tiles = (long long)tile_slice.size();
         p->xt_staged_bytes = starts * (long long)kXtCols * 4;
     }
-    if (codec == 0) {
-        // FP32: per chunk and lane {entry 2l, entry 2l+1, value 2l, value
-        // 2l+1}, one 16-byte load per lane (val is not used)
-        const size_t nl = (size_t)tile_chunk.back() * 64;
-        std::vector<uint32_t> ev(std::max<size_t>(nl, 1) * 4, 0);
+    // Per chunk and lane, the lane's two entries and its two values in one
+    // record, one load per lane (k_xtile.hip: xt_fetch_e): FP32 and BF16
+    // hi+lo 4 words, FP16 / int16 3 (the two 2-byte values in the third).
+    // FP8 keeps entries and values apart (2 bytes of values per lane).
+    const size_t nl = (size_t)tile_chunk.back() * 64;
+    auto pack = [&](int words, auto value_word) {
+        std::vector<uint32_t> ev(std::max<size_t>(nl, 1) * (size_t)words, 0);
 #pragma omp parallel for schedule(static) num_threads(omp_threads())
         for (long long i = 0; i < (long long)nl; ++i) {
-            uint32_t* o = ev.data() + (size_t)i * 4;
+            uint32_t* o = ev.data() + (size_t)i * words;
             o[0] = idx[(size_t)i * 2];
             o[1] = idx[(size_t)i * 2 + 1];
-            std::memcpy(o + 2, &val[(size_t)i * 2], 8);
+            value_word((size_t)i, o + 2);
         }
         d.idx = (const unsigned*)p->add(ctx, ev.data(), ev.size() * 4, &rc);
-        if (rc) return rc;
         d.val = d.idx;
-    } else {
-        d.idx = (const unsigned*)p->add(ctx, idx.data(), idx.size() * 4, &rc);
-        if (rc) return rc;
-    }
+    };
     std::vector<float> vscale;
     double bound = 0.0;  // row-scaled codecs: the scale cancels in the ratio
-    if (codec == kCodecBf16x2) {
+    if (codec == 0) {
+        pack(4, [&](size_t i, uint32_t* o) { std::memcpy(o, &val[i * 2], 8); });
+    } else if (codec == kCodecBf16x2) {
         std::vector<uint32_t> w(val.size());
 #pragma omp parallel for schedule(static) num_threads(omp_threads()) reduction(max : bound)
         for (long long i = 0; i < (long long)val.size(); ++i) {
             w[(size_t)i] = bf16x2_word(val[(size_t)i]);
             bound = std::max(bound, rel_err(val[(size_t)i], bf16x2_decode(w[(size_t)i])));
         }
-        d.val = p->add(ctx, w.data(), w.size() * 4, &rc);
+        pack(4, [&](size_t i, uint32_t* o) { o[0] = w[i * 2], o[1] = w[i * 2 + 1]; });
     } else if (codec_row_scaled(codec)) {
         std::vector<uint16_t> h(val.size());
 #pragma omp parallel for schedule(static) num_threads(omp_threads()) reduction(max : bound)
@@ -1041,9 +1041,11 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
             h[(size_t)i] = row_encode(codec, val[(size_t)i]);
             bound = std::max(bound, rel_err(val[(size_t)i], row_decode(codec, h[(size_t)i])));
         }
-        d.val = p->add(ctx, h.data(), std::max<size_t>(h.size(), 2) * 2, &rc);
+        pack(3, [&](size_t i, uint32_t* o) { o[0] = (uint32_t)h[i * 2] | ((uint32_t)h[i * 2 + 1] << 16); });
         vscale.swap(rscale);
     } else if (codec == kCodecFp8Slice) {
+        d.idx = (const unsigned*)p->add(ctx, idx.data(), idx.size() * 4, &rc);
+        if (rc) return rc;
         const long long nc = std::max(1, tile_chunk.back());
         std::vector<uint8_t> q((size_t)nc * kXtChunk, 0);
         vscale.assign((size_t)nc, 1.0f);
