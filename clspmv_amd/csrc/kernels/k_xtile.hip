@@ -84,8 +84,8 @@ __device__ __forceinline__ float4 x_store4(f32x4u v, int c, int cols) {
 // continues the previous chunk of the tile (`carry`): that one is stored
 // into the chunk's carry slot.  The additions are LDS read-modify-writes
 // with all N reads issued before the writes (one LDS round trip); a site
-// that adds nothing rewrites the lane's dummy slot instead, so there is no
-// branch (dummy values are never used).  The rows added to are distinct:
+// that adds nothing -- or adds padding, whose row is a dummy slot -- rewrites
+// a dummy slot instead, so there is no branch (dummy values are never used).  The rows added to are distinct:
 // within one round a row is added to by one lane of one wave only, and the
 // sites of one lane end different segments of a row-sorted chunk.
 template <int N>
@@ -98,7 +98,7 @@ __device__ __forceinline__ void xt_emit(const bool (&on)[N], const int (&row)[N]
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const bool first = on[i] && row[i] == first_row;
-        const bool direct = on[i] && row[i] != (int)kXtPadRow && !(carry && first);
+        const bool direct = on[i] && !(carry && first);
         p[i] = direct ? &y_l[row[i]] : &dummy[lane];
         fv = first ? s[i] : fv;
         fc = fc || first;
@@ -196,11 +196,10 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
         // No row crosses a lane boundary and none continues the previous
         // chunk: each lane adds its entries' sums to their rows directly --
         // entry a alone when the lane's two rows differ, the lane's tail sum
-        // to row b unless it is padding (padding sorts last, so a lane whose
-        // rows differ has a real row a).
+        // to row b (padding, whose row is a dummy slot, adds there).
         if (lane == 0) cr[wave] = -1;
         float* p1 = brk ? &y_l[head_row] : &dummy[lane];
-        float* p2 = tail_row != (int)kXtPadRow ? &y_l[tail_row] : &dummy[lane];
+        float* p2 = &y_l[tail_row];
         const float o1 = *p1, o2 = *p2;
         *p1 = o1 + head_sum;
         *p2 = o2 + tail_sum;
@@ -438,11 +437,11 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     // the metadata is read-only and unaliased, so its wave-uniform reads
     // become scalar loads that do not wait on the vector-load queue
     const XtArgs a{blk_row, blk_tile, tile_slice, tile_chunk, idx2, val, vscale, cols, x, y};
-    __shared__ float y_l[kXtRows];
+    __shared__ float y_l[kXtRows + 64];  // + one dummy slot per lane (the first is the padding row)
     __shared__ float4 x_l4[kXtX4];
     __shared__ int c_row[kXtRounds][kXtWaves];
     __shared__ float c_val[kXtRounds][kXtWaves];
-    __shared__ float dummy[64];  // one per lane: where a lane's non-emits add 0
+    float* dummy = y_l + kXtRows;  // where a lane's non-emits add
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = blockIdx.x;
     const int rb = b / groups;  // row block; column group b % groups

@@ -125,9 +125,9 @@ constexpr bool codec_row_scaled(int c) { return c == kCodecFp16Row || c == kCode
 // into 16 or 47 into 48 / from lane 31 into 32 (k_xtile.hip).
 constexpr int kXtThreads = 1024;
 constexpr int kXtWaves = kXtThreads / 64;
-// LDS: y partial sums of kXtRows rows (128 KiB) + one x slice of kXtCols
-// columns (31.5 KiB) + 2 x 16 carry slots (256 B) + 64 dummy slots (256 B)
-// = 160 KiB, all of it.
+// LDS: y partial sums of kXtRows rows (128 KiB) + 64 dummy slots (256 B) +
+// one x slice of kXtCols columns (31.5 KiB) + 2 x 16 carry slots (256 B) =
+// 160 KiB, all of it.
 constexpr int kXtRows = 32768;
 constexpr int kXtCols = 8064;
 constexpr int kXtX4 = kXtCols / 4;                                  // float4 per slice
@@ -136,7 +136,9 @@ constexpr int kXtChunk = 128;
 constexpr int kXtRounds = 2;                       // chunk rounds per tile held in registers
 constexpr int kXtTileChunks = kXtRounds * kXtWaves;  // a slice with more chunks is split
 constexpr int kXtDepth = 3;                        // tiles prefetched ahead
-constexpr unsigned kXtPadRow = 0xFFFFu;
+// Padding entries' row: the first of 64 per-lane dummy slots that follow a
+// block's kXtRows sums in LDS (written, never read).
+constexpr unsigned kXtPadRow = kXtRows;
 constexpr unsigned kXtCarryBit = 0x8000u;  // entry 0
 constexpr unsigned kXtFastBit = 0x4000u;   // entry 0
 constexpr unsigned kXtBc31Bit = 0x2000u;   // entry 0

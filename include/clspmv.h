@@ -350,7 +350,7 @@ int clspmv_part_bsr(clspmv_ctx_t ctx, const clspmv_bsr_matrix* m, int method, cl
  * word tile_slice[t] = (column slice << 2) | 2 if one of its chunks has the
  * carry bit | 1 if it continues the previous tile's slice.  Entry j of the
  * chunks: idx[j] = (row in block << 16) | column in the 8064-column slice
- * (bits 0-12), val[j] its value; padding entries have row 0xFFFF and value
+ * (bits 0-12), val[j] its value; padding entries have row 32768 and value
  * 0.  Bits 13-15 of a chunk's first two entries hold its flags: entry 0 bit
  * 15 carry (the chunk's first row continues the previous chunk of the
  * tile), bit 14 fast (no row crosses a lane boundary: entries 2l-1 and 2l

@@ -21,7 +21,7 @@ from clspmv_amd import _lib as L
 from clspmv_amd import matrix as M
 from tests.helpers import rand_coo
 
-CHUNK, XT_COLS, PAD = 128, 8064, 0xFFFF
+CHUNK, XT_COLS, PAD = 128, 8064, 32768
 CARRY, FAST, BC31 = 0x8000, 0x4000, 0x2000  # entry 0 of a chunk
 BC15 = 0x8000                                # entry 1; bits 13-14: steps - 1
 COLMASK = 0x1FFF
