@@ -151,7 +151,7 @@ static_assert(kXtRounds * kXtWaves == 32, "the carry slots of a tile fill lanes 
 // order; row -1 marks an unused slot, as do rounds not in `pending`), runs
 // of equal rows are summed by a segmented scan in DPP moves (row shifts,
 // then lane 15 into lanes 16..31), and the last lane of each run adds the
-// run's sum to its row -- one LDS read-modify-write per row.
+// run's sum to its row -- one LDS addition per row.
 __device__ __forceinline__ void xt_apply_carries(int pending, int (*c_row)[kXtWaves], float (*c_val)[kXtWaves],
                                                  float* y_l, int lane) {
     const int k = (lane >> 4) & 1, i = lane & 15;  // lanes 32..63 mirror 0..31 (unused)
@@ -168,7 +168,10 @@ __device__ __forceinline__ void xt_apply_carries(int pending, int (*c_row)[kXtWa
     seg_step((lane & 16) != 0, dpp_f<0x142, 0xa>(v), dpp_i<0x142, 0xa>(flag), v, flag);
     // the run's last slot: the next lane holds another row (lanes 31.. end)
     const int r_next = __builtin_amdgcn_update_dpp(-1, r, 0x130, 0xf, 0xf, false);  // lane + 1 (wave shift left by one)
-    if (lane < 32 && r >= 0 && (lane == 31 || r_next != r)) y_l[r] += v;
+    // an LDS add without return: the wave does not wait for y_l's old value
+    // (no other wave touches y_l until the next barrier, and the rows are
+    // distinct, so the sums are those of a read-modify-write)
+    if (lane < 32 && r >= 0 && (lane == 31 || r_next != r)) atomicAdd(&y_l[r], v);
 }
 
 // One wavefront reduces one 128-entry chunk (2 entries per lane).  Each
