@@ -180,9 +180,12 @@ __device__ __forceinline__ void xt_apply_carries(int pending, int (*c_row)[kXtWa
 // rows differ, its tail row when the next lane does not go on with it), all
 // evaluated without branches.  The chunk's first row goes to the chunk's
 // carry slot when its first entry has the carry bit (the row continues the
-// previous chunk of the tile), to y_l otherwise.
+// previous chunk of the tile), to y_l otherwise.  With `scan_prio` (set
+// for layouts whose chunks are mostly fast) a wave on a scan chunk raises
+// its issue priority for the chunk: those waves are the ones the next
+// barrier waits for.
 __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, float* y_l, float* dummy, int* cr,
-                                         float* cv, int wave, int lane, bool tile_carry) {
+                                         float* cv, int wave, int lane, bool tile_carry, bool scan_prio) {
     // products rounded, then added: no multiply-add contraction, so that
     // every value codec (and every compilation) sums the same FP32 numbers
 #pragma clang fp contract(off)
@@ -208,6 +211,7 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
         *p2 = o2 + tail_sum;
         return;
     }
+    if (scan_prio) __builtin_amdgcn_s_setprio(1);
     const int first_row = __builtin_amdgcn_readfirstlane(head_row);
     const bool carry = (flags & (int)kXtCarryBit) != 0;
     if (tile_carry && lane == 0) cr[wave] = carry ? first_row : -1;
@@ -240,6 +244,7 @@ __device__ __forceinline__ void xt_chunk(u32x2 e, f32x2 v, const float* x_l, flo
     const int row[2] = {head_row, tail_row};
     const float sum[2] = {head_sum + (cont ? prev_c : 0.f), c};
     xt_emit<2>(on, row, sum, first_row, carry, y_l, dummy, slot, lane);
+    if (scan_prio) __builtin_amdgcn_s_setprio(0);
 }
 
 // Per-lane value storage of one chunk (two entries) for each codec.
@@ -286,6 +291,7 @@ struct XtArgs {
     const void* val;      // FP8: chunk values, XtVal<CODEC>::T per lane (the other codecs' are in idx2's records)
     const float* vscale;  // FP16: row scales; FP8: chunk scales
     int cols;
+    int scan_prio;  // xt_chunk
     const float* x;
     float* y;
 };
@@ -423,7 +429,7 @@ __device__ __forceinline__ void xt_step(XtSet<CODEC>& S, XtMeta& M, const XtArgs
         if (cb < c1) {
             if (cb + wave < c1)
                 xt_chunk(S.e[k], xt_decode<CODEC>(S.v[k], S.s[k]), x_l, y_l, dummy, c_row[k], c_val[k], wave, lane,
-                         (ts & 2) != 0);
+                         (ts & 2) != 0, a.scan_prio != 0);
             else if ((ts & 2) && lane == 0)
                 c_row[k][wave] = -1;
             if (ts & 2) st.pending |= 1 << k;  // the tile has a carry
@@ -437,10 +443,10 @@ __global__ __launch_bounds__(kXtThreads) void xtile_kernel(
     const int* __restrict__ blk_row, const int* __restrict__ blk_tile, const int* __restrict__ tile_slice,
     const int* __restrict__ tile_chunk, const u32x2* __restrict__ idx2, const void* __restrict__ val,
     const float* __restrict__ vscale, int cols, const float* __restrict__ x, float* __restrict__ y,
-    float* __restrict__ partial, int groups, int rows) {
+    float* __restrict__ partial, int groups, int rows, int scan_prio) {
     // the metadata is read-only and unaliased, so its wave-uniform reads
     // become scalar loads that do not wait on the vector-load queue
-    const XtArgs a{blk_row, blk_tile, tile_slice, tile_chunk, idx2, val, vscale, cols, x, y};
+    const XtArgs a{blk_row, blk_tile, tile_slice, tile_chunk, idx2, val, vscale, cols, scan_prio, x, y};
     __shared__ float y_l[kXtRows + 64];  // + one dummy slot per lane (the first is the padding row)
     __shared__ float4 x_l4[kXtX4];
     __shared__ int c_row[kXtRounds][kXtWaves];
@@ -525,7 +531,7 @@ int launch_xtile(const DevXtile& m, const float* x, float* y, hipStream_t st) {
 #define CLSPMV_XTILE(C)                                                                                          \
     hipLaunchKernelGGL(dev::xtile_kernel<C>, dim3(m.nblocks), dim3(kXtThreads), 0, st, m.blk_row, m.blk_tile, \
                        m.tile_slice, m.tile_chunk, reinterpret_cast<const dev::u32x2*>(m.idx), m.val, m.vscale,   \
-                       m.cols, x, y, m.partial, m.groups, m.rows)
+                       m.cols, x, y, m.partial, m.groups, m.rows, m.scan_prio)
     switch (m.codec) {
         case kCodecBf16x2: CLSPMV_XTILE(kCodecBf16x2); break;
         case kCodecFp16Row: CLSPMV_XTILE(kCodecFp16Row); break;

@@ -162,6 +162,7 @@ struct DevXtile {
     int groups;             // column groups (1..8); workgroup b works on group b % groups
     int ntiles, nchunks, cols, rows;
     int codec;              // 0 (FP32) or a kCodec* value
+    int scan_prio;          // 1: waves on scan chunks raise their issue priority (k_xtile.hip: xt_chunk)
 };
 
 struct DevDiaMixed {

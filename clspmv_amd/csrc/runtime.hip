@@ -781,6 +781,9 @@ void xt_walk_rows(const int* rp, const int* ci, int r0, int r1, std::vector<int>
 
 // Column groups per row block at most (tunable: "xtile_max_groups", 1..8).
 int g_xt_max_groups = kXtMaxGroups;
+// Issue priority of the x-tiled kernel's waves on scan chunks: -1 by the
+// layout's share of fast chunks, else 0 or 1 (tunable: "xtile_scan_prio").
+int g_xt_scan_prio = -1;
 
 // Builds the LDS x-tiled layout (DevXtile) from row lists.  Every row block
 // stages the x it touches once, so the x traffic is (row blocks) x (x
@@ -807,6 +810,7 @@ struct XtHostLayout {
     std::vector<unsigned> idx;
     std::vector<float> val, rscale;
     int nb = 0, G = 1;
+    long long fast_chunks = 0;  // chunks with the fast bit
 };
 
 int xt_host_layout(const int* rp, const int* ci, const float* cv, int rows, int cols, int codec, XtHostLayout* out) {
@@ -938,7 +942,8 @@ int xt_host_layout(const int* rp, const int* ci, const float* cv, int rows, int 
     // a 16-lane row), at least 1, and whether a run crosses lanes 15|16 or
     // 47|48 (bc15) and 31|32 (bc31).  A tile holding a carry gets bit 1 of
     // its slice word.
-#pragma omp parallel for schedule(static) num_threads(omp_threads())
+    long long fast_chunks = 0;
+#pragma omp parallel for schedule(static) num_threads(omp_threads()) reduction(+ : fast_chunks)
     for (long long t = 0; t < (long long)tile_slice.size(); ++t) {
         bool carries = false;
         for (int c = tile_chunk[(size_t)t]; c < tile_chunk[(size_t)t + 1]; ++c) {
@@ -952,6 +957,7 @@ int xt_host_layout(const int* rp, const int* ci, const float* cv, int rows, int 
                 carries = true;
             } else if (fast) {
                 idx[q] |= kXtFastBit;
+                ++fast_chunks;
                 continue;
             }
             auto same_tail = [&](int l) { return row(2 * l + 1) != kXtPadRow && row(2 * l + 1) == row(2 * l - 1); };
@@ -977,6 +983,7 @@ int xt_host_layout(const int* rp, const int* ci, const float* cv, int rows, int 
     out->rscale.swap(rscale);
     out->nb = nb;
     out->G = G;
+    out->fast_chunks = fast_chunks;
     return CLSPMV_OK;
 }
 
@@ -1071,6 +1078,10 @@ int build_xtile(clspmv_ctx_t ctx, clspmv_part_t p, const int* rp, const int* ci,
         if (rc) return rc;
     }
     d.codec = codec;
+    // when at least half the chunks take the fast path, the waves on scan
+    // chunks are the ones each barrier waits for: they raise their issue
+    // priority (tunable: "xtile_scan_prio" -1 this rule, 0 never, 1 always)
+    d.scan_prio = g_xt_scan_prio >= 0 ? g_xt_scan_prio : (2 * H.fast_chunks >= (long long)tile_chunk.back() ? 1 : 0);
     if (G > 1) {
         d.partial = (float*)p->scratch((size_t)G * rows * 4, &rc);
         if (rc) return rc;
@@ -1810,6 +1821,11 @@ int clspmv_set_option(const char* name, long long value) {
     if (std::strcmp(name, "xtile_max_groups") == 0) {
         if (value < 1 || value > kXtMaxGroups) return CLSPMV_E_INVALID;
         g_xt_max_groups = (int)value;
+        return CLSPMV_OK;
+    }
+    if (std::strcmp(name, "xtile_scan_prio") == 0) {
+        if (value < -1 || value > 1) return CLSPMV_E_INVALID;
+        g_xt_scan_prio = (int)value;
         return CLSPMV_OK;
     }
     if (std::strcmp(name, "bsr_lds_window") == 0) {

@@ -412,6 +412,11 @@ int clspmv_part_spmv(clspmv_part_t p, const float* d_x, float* d_y, int ntimes);
  *   "xtile_max_groups"  column groups per row block of the x-tiled layout
  *                       at most, 1..8 (default 8; with > 1 group a
  *                       partial-sum pass adds the groups' sums to y);
+ *   "xtile_scan_prio"   -1 (default): in the x-tiled kernel a wave on a
+ *                       scan chunk (one where a row crosses a lane) raises
+ *                       its issue priority for that chunk when at least half
+ *                       the layout's chunks are fast ones; 0: never; 1:
+ *                       always (the results are the same bits either way);
  *   "cocktail_merge_flat"  1 (default): clspmv_init_mat_kernels runs the
  *                       flat parts (ELL, SELL, CSR, COO) of a partition as
  *                       one merged row-list part; 0: one part each;

@@ -148,6 +148,27 @@ def test_xtile_block_counts(ctx, xt_blocks, nblocks, name, groups):
     _check(coo, y, x, y0, f"xtile blocks={nblocks} {name} {info['kernel']}")
 
 
+@pytest.mark.parametrize("name", ["uniform_rows", "zipf", "exact_chunks", "groups4", "rows_over_block_cap"])
+def test_xtile_scan_prio_same_bits(ctx, name):
+    # the issue priority of the waves on scan chunks (xtile_scan_prio 0 / 1
+    # / the layout's choice) changes no sum: y is the same bits
+    coo = dict(CASES)[name]()
+    x = M.gen_vector(coo.width, 23)
+    y0 = M.gen_vector(coo.height, 24)
+    csr = M.coo2csr(coo)
+    ys = []
+    try:
+        for prio in (0, 1, -1):
+            L.check(L.lib().clspmv_set_option(b"xtile_scan_prio", prio), "set_option")
+            y, _ = gpu_spmv(ctx, L.FMT_CSR_VECTOR, csr, x, y0, XT)
+            ys.append(y)
+    finally:
+        L.check(L.lib().clspmv_set_option(b"xtile_scan_prio", -1), "set_option")
+    assert np.array_equal(ys[0].view(np.uint32), ys[1].view(np.uint32)), name
+    assert np.array_equal(ys[0].view(np.uint32), ys[2].view(np.uint32)), name
+    _check(coo, ys[1], x, y0, f"xtile scan prio 1 {name}")
+
+
 def test_xtile_repeatable(ctx):
     coo = M.gen_zipf_rows(60000, 1.8, 16.0, 20000, 12)
     x = M.gen_vector(coo.width, 1)
